@@ -1,0 +1,319 @@
+/*
+ * lyap.h - C ABI of the MI355X Lyapunov library (liblyap_amd.so).
+ *
+ * This is the drop-in boundary for the reference's device path. Each entry
+ * names the reference interface it replaces. Only plain C types and the PODs
+ * of lyap_types.h cross it: no HIP, CUDA or torch type appears here.
+ *
+ * Conventions
+ *  - Every function that can fail returns an int status: LYAP_OK (0) or a
+ *    negative LYAP_ERR_* code. Nothing in the library calls exit(); the
+ *    reference's helpers print and exit (helper_cuda.h, scene.cu:98-100).
+ *    lyap_strerror() names a code and lyap_last_error() gives the message of
+ *    the last failure on a context.
+ *  - Device buffers are allocated and owned by the caller, as in the
+ *    reference (lyap_interactive.cu:115-116, lyap_calculate.cu:64-67).
+ *    Parameter structs and the sequence are copied at call time, so the
+ *    caller may change them as soon as the call returns (the reference passes
+ *    them to the kernel by value).
+ *  - Compute calls are asynchronous on the context's stream; lyap_sync()
+ *    waits. A context is used from one host thread at a time.
+ *  - Sequences are host arrays of letter indices 0..3 (A..D) terminated by
+ *    -1, as scene_convert_sequence produces them (scene.cu:69-108). Their
+ *    period (entries before the -1) is 1..LYAP_MAX_SEQ.
+ *  - Sizes need not be multiples of any block size: every kernel checks its
+ *    bounds (the reference requires multiples of 16 or 8).
+ */
+#ifndef LYAP_H
+#define LYAP_H
+
+#include "lyap_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LYAP_ABI_VERSION 1
+#define LYAP_MAX_SEQ 32     /* longest sequence period accepted */
+#define LYAP_MAX_LIGHTS 16  /* params.hpp:21 */
+
+enum {
+    LYAP_OK = 0,
+    LYAP_ERR_INVALID_ARGUMENT = -1,
+    LYAP_ERR_BAD_SEQUENCE = -2,   /* letter outside A-D/a-d/1-9, empty, too long */
+    LYAP_ERR_UNSUPPORTED = -3,
+    LYAP_ERR_NO_DEVICE = -4,
+    LYAP_ERR_DEVICE = -5,         /* a HIP runtime call failed */
+    LYAP_ERR_BUDGET = -6,         /* request does not fit the HBM budget */
+    LYAP_ERR_QUEUE_FULL = -7,
+    LYAP_ERR_NOT_FOUND = -8,
+    LYAP_ERR_CANCELLED = -9,
+    LYAP_ERR_PENDING = -10,       /* lyap_poll: request not finished yet */
+    LYAP_ERR_NO_LIBRARY = -11     /* reserved for bindings that fail to load the library */
+};
+
+const char *lyap_strerror(int status);
+
+/* ------------------------------------------------------------------------ */
+/* Host scene helpers (no device needed).                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Defaults of params_init() (params.cu:21-114): parameters, the scripted
+ * camera and the single default spotlight. Any pointer may be NULL. `lights`
+ * must hold LYAP_MAX_LIGHTS entries. The camera's derived vectors are not
+ * computed: call lyap_scene_cam_recalculate() for a render size. */
+void lyap_params_init(LyapParams *prm, LyapCam *cam, LyapLight *lights,
+                      unsigned *num_lights);
+
+/* The default sequence string of params_init, "BCABA" (params.cu:39). */
+const char *lyap_default_sequence(void);
+
+/* scene_convert_sequence (scene.cu:69-108): "A".."D" (either case) map to
+ * 0..3; a digit n appends n more copies of the previous letter (B before any
+ * letter); a -1 terminator is appended. Writes at most `cap` ints and stores
+ * the length including the terminator in *out_len. A bad letter returns
+ * LYAP_ERR_BAD_SEQUENCE instead of exit(1). */
+int lyap_scene_convert_sequence(const char *str, int32_t *out, size_t cap,
+                                size_t *out_len);
+
+/* scene_cam_recalculate (scene.cu:31-63): clamps M, normalises Q, sets
+ * V, S0, SDX, SDY and the render size. tw/th/td of 0 keep the old value. */
+void lyap_scene_cam_recalculate(LyapCam *cam, unsigned tw, unsigned th,
+                                unsigned td);
+
+/* scene_lights_recalculate (scene.cu:20-29): spotlight axis and cone cosines. */
+void lyap_scene_lights_recalculate(LyapLight *lights, size_t num_lights);
+
+/* QUAT(axis, angle, inDegrees) (quat.hpp:227-238). */
+void lyap_quat_axis_angle(const LyapVec *axis, float angle, int in_degrees,
+                          LyapQuat *out);
+/* QUAT(p, q, scale) (quat.hpp:194-220): rotation taking unit p toward unit q,
+ * scaled by `scale` of the angle. */
+void lyap_quat_between(const LyapVec *p, const LyapVec *q, float scale,
+                       LyapQuat *out);
+/* QUAT::transform (quat.hpp:279-284). */
+void lyap_quat_transform(const LyapQuat *q, const LyapVec *v, LyapVec *out);
+
+/* Camera at `eye` looking at `target` (new; used by the multi-view
+ * workloads): Q is the rotation of +z onto the view direction, M the screen
+ * half-width, then lyap_scene_cam_recalculate(w, h, 1). */
+void lyap_cam_look_at(LyapCam *cam, const LyapVec *eye, const LyapVec *target,
+                      float M, unsigned w, unsigned h);
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct lyap_ctx lyap_ctx;
+
+/* Opens `device`, creates a non-blocking stream at default priority.
+ * hbm_budget_bytes caps the library's own device allocations (scratch,
+ * queues, server state); 0 means no cap. */
+int lyap_create(lyap_ctx **out, int device, size_t hbm_budget_bytes);
+void lyap_destroy(lyap_ctx *ctx);
+int lyap_sync(lyap_ctx *ctx);
+/* The hipStream_t the context launches on, as an opaque pointer. */
+void *lyap_get_stream(lyap_ctx *ctx);
+const char *lyap_last_error(lyap_ctx *ctx);
+/* Bytes of device memory the library currently holds for this context. */
+size_t lyap_bytes_in_use(lyap_ctx *ctx);
+
+/* Convenience device memory helpers for callers without a HIP toolchain
+ * (the reference's drivers call cudaMalloc/cudaMemcpy directly). */
+int lyap_device_alloc(lyap_ctx *ctx, void **ptr, size_t bytes);
+int lyap_device_free(lyap_ctx *ctx, void *ptr);
+int lyap_copy_to_device(lyap_ctx *ctx, void *dst, const void *src, size_t bytes);
+int lyap_copy_to_host(lyap_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Arithmetic forms of the lambda iteration                                  */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    /* FP32 iterate and one log per accumulated step: the reference's method
+     * (kernel.cu:140-150). */
+    LYAP_PREC_PLAIN = 0,
+    /* FP32 iterate in u = 1 - 2x, derivative product carried as an FP32
+     * mantissa with an integer exponent, renormalised every `renorm` steps;
+     * one log per evaluation. */
+    LYAP_PREC_PRODUCT = 1,
+    /* FP64 iterate with the reference's operation order (DOUBLE_REALS build). */
+    LYAP_PREC_FP64 = 2
+};
+
+/* Volume options. NULL selects {LYAP_PREC_PLAIN, no re-evaluation}. */
+typedef struct LyapVolumeMode {
+    int32_t precision;      /* LYAP_PREC_* */
+    uint32_t renorm;        /* PRODUCT: steps between renormalisations, 0 = 16 */
+    float fp64_margin;      /* re-evaluate in FP64 where |lambda| < margin; 0 = off */
+    float converge_tau;     /* also re-evaluate where the two halves of the window */
+    float converge_band;    /*   differ by > tau and the smaller half-lambda < band */
+    uint32_t reserved[3];
+} LyapVolumeMode;
+
+/* kernel_calc_volume (kernel.hpp:19, kernel.cu:518-532; launched at
+ * lyap_calculate.cu:72): lambda of every voxel of an nx*ny*nz grid over the
+ * box [lo, hi) with coordinate lo + (hi-lo)*i/n per axis and index
+ * x + (y + z*ny)*nx. The reference's box is lo = 0, hi = 4. */
+int lyap_calc_volume(lyap_ctx *ctx, float *d_exps, const LyapParams *prm,
+                     const int32_t *seq, const float lo[3], const float hi[3],
+                     unsigned nx, unsigned ny, unsigned nz,
+                     const LyapVolumeMode *mode);
+
+/* lambda at n arbitrary points (x, y, z float triples in device memory):
+ * lyap4d (kernel.hpp:13, kernel.cu:108-154) as a batch entry. */
+int lyap_calc_points(lyap_ctx *ctx, float *d_out, const float *d_xyz, size_t n,
+                     const LyapParams *prm, const int32_t *seq,
+                     const LyapVolumeMode *mode);
+
+/* Statistics of the last volume/points call on this context: voxels whose
+ * lambda was re-evaluated in FP64. Waits for that call to finish. */
+int lyap_last_reevaluated(lyap_ctx *ctx, uint64_t *count);
+
+/* ------------------------------------------------------------------------ */
+/* Render                                                                    */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    LYAP_REFINE_WALK = 0,   /* reversing half-step walk (kernel.cu:402-441) */
+    LYAP_REFINE_BISECT = 1  /* fixed-count bisection of the last march step */
+};
+
+/* Render options. NULL selects the reference's method: refinement walk,
+ * normal offset step*gradient, incremental positions, LYAP_PREC_PLAIN. */
+typedef struct LyapRenderMode {
+    int32_t refine;         /* LYAP_REFINE_* */
+    int32_t bisect_steps;   /* BISECT: number of halvings */
+    float normal_h;         /* > 0: absolute normal offset; 0: step * gradient */
+    int32_t precision;      /* LYAP_PREC_PLAIN or LYAP_PREC_PRODUCT */
+    float fp64_margin;      /* BISECT: samples with |lambda - opaque| < margin
+                               are re-evaluated in FP64; 0 = off */
+    uint32_t lanes_per_ray; /* 0 = chosen by the library */
+    uint32_t reserved[2];
+} LyapRenderMode;
+
+/* kernel_calc_render (kernel.hpp:17, kernel.cu:500-516; launched at
+ * lyap_interactive.cu:711): one primary ray per pixel of a w*h image through
+ * the box [lMin, lMax]^3, hit record into d_points[x + y*w] and shaded
+ * colour into d_rgba. A miss writes P = N = 0, a = l = NaN, c = 0 and colour
+ * (0,0,0,0) (the reference leaves the record unwritten). d_lights is a device
+ * array of num_lights <= 16 lights (lyap_interactive.cu:115,130). d_depth and
+ * d_lambda are optional per-pixel outputs (ray parameter t and lambda of the
+ * hit, NaN on a miss); d_evals optionally receives the number of lambda
+ * evaluations each ray's method needed. Any output pointer except d_rgba may
+ * be NULL. */
+int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_points,
+                     const LyapCam *cam, const LyapParams *prm,
+                     const int32_t *seq, const LyapLight *d_lights,
+                     unsigned num_lights, unsigned w, unsigned h,
+                     float *d_depth, float *d_lambda, uint32_t *d_evals,
+                     const LyapRenderMode *mode);
+
+/* Total algorithmic lambda evaluations of the last render/batch call
+ * (march samples through the first opaque one, refinement, normal). Waits
+ * for that call to finish. */
+int lyap_last_evals(lyap_ctx *ctx, uint64_t *count);
+
+/* ------------------------------------------------------------------------ */
+/* Batched views: many cameras in one persistent launch                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct LyapView {
+    LyapCam cam;                       /* derived vectors already computed */
+    LyapParams prm;
+    int32_t seq[LYAP_MAX_SEQ + 1];     /* -1 terminated */
+    uint32_t width, height;
+    LyapRGBA *rgba;                    /* device outputs, as in lyap_calc_render */
+    LyapPoint *points;
+    float *depth;
+    float *lambda;
+    uint32_t *evals;
+    const LyapLight *lights;           /* device array */
+    uint32_t num_lights;
+    uint32_t reserved;
+} LyapView;
+
+/* Launch shape of the persistent kernels. Zero fields are filled from the
+ * autotuner's table (lyap_autotune) or built-in defaults. */
+typedef struct LyapTuning {
+    uint32_t tile_rays;        /* rays handed out per work-queue claim */
+    uint32_t waves_per_cu;     /* resident waves requested per CU */
+    uint32_t lanes_per_ray;    /* lanes sharing one ray's march */
+    uint32_t reserved;
+} LyapTuning;
+
+/* Renders n_views views in one persistent launch fed by a device work queue
+ * of ray tiles. Views may differ in size, camera, parameters and sequence. */
+int lyap_render_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                      const LyapRenderMode *mode, const LyapTuning *tuning);
+
+/* Times candidate launch shapes on a representative batch and stores the
+ * fastest in the context (used when a LyapTuning field is 0). */
+int lyap_autotune(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                  const LyapRenderMode *mode, LyapTuning *best);
+
+/* A captured frame: the batch upload and the persistent launch recorded once
+ * as a HIP graph and replayed. The views' device buffers must stay valid. */
+typedef struct lyap_frame lyap_frame;
+int lyap_frame_capture(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                       const LyapRenderMode *mode, const LyapTuning *tuning,
+                       lyap_frame **out);
+int lyap_frame_replay(lyap_ctx *ctx, lyap_frame *frame);
+void lyap_frame_destroy(lyap_frame *frame);
+
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching of render and volume requests               */
+/* ------------------------------------------------------------------------ */
+
+enum { LYAP_REQ_VIEW = 0, LYAP_REQ_VOLUME = 1 };
+
+typedef struct LyapRequest {
+    int32_t kind;                      /* LYAP_REQ_* */
+    uint32_t priority;                 /* reserved, 0 */
+    LyapView view;                     /* VIEW: outputs may be NULL: the server
+                                          then keeps RGBA and lambda resident */
+    LyapParams vol_prm;                /* VOLUME */
+    int32_t vol_seq[LYAP_MAX_SEQ + 1];
+    float vol_lo[3], vol_hi[3];
+    uint32_t vol_n[3];
+    float *vol_out;                    /* VOLUME: device output or NULL */
+    double deadline_ms;                /* relative deadline used for ordering */
+} LyapRequest;
+
+typedef struct LyapServerConfig {
+    size_t hbm_budget_bytes;           /* admission cap for resident state */
+    uint32_t max_inflight;             /* launches queued on the device */
+    uint32_t slice_tiles;              /* tiles per launch (0: derived) */
+    LyapRenderMode mode;
+    LyapVolumeMode vol_mode;
+} LyapServerConfig;
+
+typedef struct LyapResult {
+    int32_t status;                    /* LYAP_OK, LYAP_ERR_PENDING, ... */
+    uint32_t kind;
+    double submit_ms, start_ms, finish_ms;   /* server clock */
+    uint64_t rays, voxels, evals;
+    void *rgba;                        /* resident outputs (device) */
+    float *lambda;
+} LyapResult;
+
+typedef struct lyap_server lyap_server;
+int lyap_server_create(lyap_ctx *ctx, const LyapServerConfig *cfg,
+                       lyap_server **out);
+void lyap_server_destroy(lyap_server *srv);
+/* Queues a request; admission against the budget happens when it starts. */
+int lyap_server_submit(lyap_server *srv, const LyapRequest *req,
+                       uint64_t *ticket);
+/* Advances the server: retires finished launches, admits requests, and
+ * issues new launches. Never blocks longer than `wait_ms`. */
+int lyap_server_progress(lyap_server *srv, double wait_ms);
+int lyap_server_poll(lyap_server *srv, uint64_t ticket, LyapResult *out);
+int lyap_server_release(lyap_server *srv, uint64_t ticket);
+int lyap_server_cancel(lyap_server *srv, uint64_t ticket);
+double lyap_server_clock_ms(lyap_server *srv);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* LYAP_H */

@@ -1,0 +1,440 @@
+"""Python binding of liblyap_amd.so, the MI355X Lyapunov library.
+
+The library's C ABI (include/lyap.h) is the product; this module mirrors its
+types with ctypes and wraps its entries for tests, the bench and Python
+callers. Device buffers are passed as integer device addresses (for example
+``tensor.data_ptr()`` of a torch tensor on ``cuda:0``): torch is only used as
+an allocator and for streams, never for the computation.
+
+The reference surface this mirrors: structs.hpp (types), params.cu
+(params_init), scene.cu (scene_convert_sequence, scene_cam_recalculate,
+scene_lights_recalculate), kernel.hpp (kernel_calc_render,
+kernel_calc_volume).
+
+There is no CPU fallback: if the shared library is missing, ``load()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblyap_amd.so")
+
+MAX_SEQ = 32
+MAX_LIGHTS = 16
+
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_BAD_SEQUENCE = -2
+ERR_UNSUPPORTED = -3
+ERR_NO_DEVICE = -4
+ERR_DEVICE = -5
+ERR_BUDGET = -6
+ERR_QUEUE_FULL = -7
+ERR_NOT_FOUND = -8
+ERR_CANCELLED = -9
+ERR_PENDING = -10
+
+PREC_PLAIN = 0
+PREC_PRODUCT = 1
+PREC_FP64 = 2
+
+REFINE_WALK = 0
+REFINE_BISECT = 1
+
+REQ_VIEW = 0
+REQ_VOLUME = 1
+
+
+# ---------------------------------------------------------------- types ----
+
+class Vec(C.Structure):
+    """Vec (vec3.hpp:12): 12 bytes."""
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Quat(C.Structure):
+    """Quat (quat.hpp:17): 16 bytes; 16-byte aligned in C (padding is explicit
+    in the structs below, since ctypes here has no alignment attribute)."""
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class Color(C.Structure):
+    """Color (color.hpp:12): 16 bytes; 16-byte aligned in C."""
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class LyapCam(C.Structure):
+    """LyapCam / LyapLight (structs.hpp:26-51): 224 bytes."""
+    _fields_ = [
+        ("C", Vec), ("_pad0", C.c_uint32), ("Q", Quat), ("M", C.c_float), ("V", Vec),
+        ("S0", Vec), ("SDX", Vec), ("SDY", Vec),
+        ("textureWidth", C.c_uint32), ("textureHeight", C.c_uint32),
+        ("renderWidth", C.c_uint32), ("renderHeight", C.c_uint32),
+        ("renderDenominator", C.c_uint32),
+        ("lightInnerCone", C.c_float), ("lightOuterCone", C.c_float),
+        ("lightRange", C.c_float), ("_pad1", C.c_uint32 * 3),
+        ("ambient", Color), ("diffuseColor", Color), ("diffusePower", C.c_float),
+        ("_pad2", C.c_uint32 * 3),
+        ("specularColor", Color), ("specularPower", C.c_float),
+        ("specularHardness", C.c_float), ("_pad3", C.c_uint32 * 2), ("chaosColor", Color),
+    ]
+
+
+LyapLight = LyapCam
+
+
+class LyapParams(C.Structure):
+    """LyapParams (structs.hpp:53-68): 56 bytes."""
+    _fields_ = [
+        ("d", C.c_float), ("settle", C.c_uint32), ("accum", C.c_uint32),
+        ("stepMethod", C.c_uint32), ("nearThreshold", C.c_float),
+        ("nearMultiplier", C.c_float), ("opaqueThreshold", C.c_float),
+        ("chaosThreshold", C.c_float), ("depth", C.c_float), ("jitter", C.c_float),
+        ("refine", C.c_float), ("gradient", C.c_float), ("lMin", C.c_float),
+        ("lMax", C.c_float),
+    ]
+
+
+class LyapPoint(C.Structure):
+    """LyapPoint (structs.hpp:70-76): 36 bytes."""
+    _fields_ = [("P", Vec), ("N", Vec), ("a", C.c_float), ("c", C.c_float), ("l", C.c_float)]
+
+
+class RGBA(C.Structure):
+    _fields_ = [("r", C.c_ubyte), ("g", C.c_ubyte), ("b", C.c_ubyte), ("a", C.c_ubyte)]
+
+
+class VolumeMode(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("renorm", C.c_uint32), ("fp64_margin", C.c_float),
+        ("converge_tau", C.c_float), ("converge_band", C.c_float),
+        ("reserved", C.c_uint32 * 3),
+    ]
+
+
+class RenderMode(C.Structure):
+    _fields_ = [
+        ("refine", C.c_int32), ("bisect_steps", C.c_int32), ("normal_h", C.c_float),
+        ("precision", C.c_int32), ("fp64_margin", C.c_float),
+        ("lanes_per_ray", C.c_uint32), ("reserved", C.c_uint32 * 2),
+    ]
+
+
+class View(C.Structure):
+    _fields_ = [
+        ("cam", LyapCam), ("prm", LyapParams), ("seq", C.c_int32 * (MAX_SEQ + 1)),
+        ("width", C.c_uint32), ("height", C.c_uint32),
+        ("rgba", C.c_void_p), ("points", C.c_void_p), ("depth", C.c_void_p),
+        ("lambda_", C.c_void_p), ("evals", C.c_void_p), ("lights", C.c_void_p),
+        ("num_lights", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("tile_rays", C.c_uint32), ("waves_per_cu", C.c_uint32),
+        ("lanes_per_ray", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("priority", C.c_uint32), ("_pad0", C.c_uint32 * 2), ("view", View),
+        ("vol_prm", LyapParams), ("vol_seq", C.c_int32 * (MAX_SEQ + 1)),
+        ("vol_lo", C.c_float * 3), ("vol_hi", C.c_float * 3), ("vol_n", C.c_uint32 * 3),
+        ("vol_out", C.c_void_p), ("deadline_ms", C.c_double),
+    ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_size_t), ("max_inflight", C.c_uint32),
+        ("slice_tiles", C.c_uint32), ("mode", RenderMode), ("vol_mode", VolumeMode),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("kind", C.c_uint32), ("submit_ms", C.c_double),
+        ("start_ms", C.c_double), ("finish_ms", C.c_double), ("rays", C.c_uint64),
+        ("voxels", C.c_uint64), ("evals", C.c_uint64), ("rgba", C.c_void_p),
+        ("lambda_", C.c_void_p),
+    ]
+
+
+assert C.sizeof(LyapParams) == 56
+assert C.sizeof(LyapPoint) == 36
+assert C.sizeof(LyapCam) == 224
+assert C.sizeof(RGBA) == 4
+assert LyapCam.ambient.offset == 128 and LyapCam.chaosColor.offset == 208
+assert C.sizeof(View) == 480 and C.sizeof(Request) == 736
+
+
+# ------------------------------------------------------------- library ----
+
+class LyapError(RuntimeError):
+    def __init__(self, status: int, message: str = ""):
+        self.status = status
+        super().__init__(f"lyap status {status}: {message}")
+
+
+_lib: Optional[C.CDLL] = None
+
+_P = C.c_void_p
+_SIGS = {
+    "lyap_strerror": (C.c_char_p, [C.c_int]),
+    "lyap_params_init": (None, [C.POINTER(LyapParams), C.POINTER(LyapCam), C.POINTER(LyapLight), C.POINTER(C.c_uint)]),
+    "lyap_scene_convert_sequence": (C.c_int, [C.c_char_p, C.POINTER(C.c_int32), C.c_size_t, C.POINTER(C.c_size_t)]),
+    "lyap_scene_cam_recalculate": (None, [C.POINTER(LyapCam), C.c_uint, C.c_uint, C.c_uint]),
+    "lyap_scene_lights_recalculate": (None, [C.POINTER(LyapLight), C.c_size_t]),
+    "lyap_quat_axis_angle": (None, [C.POINTER(Vec), C.c_float, C.c_int, C.POINTER(Quat)]),
+    "lyap_quat_between": (None, [C.POINTER(Vec), C.POINTER(Vec), C.c_float, C.POINTER(Quat)]),
+    "lyap_quat_transform": (None, [C.POINTER(Quat), C.POINTER(Vec), C.POINTER(Vec)]),
+    "lyap_cam_look_at": (None, [C.POINTER(LyapCam), C.POINTER(Vec), C.POINTER(Vec), C.c_float, C.c_uint, C.c_uint]),
+    "lyap_default_sequence": (C.c_char_p, []),
+    "lyap_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_size_t]),
+    "lyap_destroy": (None, [_P]),
+    "lyap_sync": (C.c_int, [_P]),
+    "lyap_get_stream": (_P, [_P]),
+    "lyap_last_error": (C.c_char_p, [_P]),
+    "lyap_bytes_in_use": (C.c_size_t, [_P]),
+    "lyap_device_alloc": (C.c_int, [_P, C.POINTER(_P), C.c_size_t]),
+    "lyap_device_free": (C.c_int, [_P, _P]),
+    "lyap_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "lyap_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "lyap_calc_volume": (C.c_int, [_P, _P, C.POINTER(LyapParams), C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint, C.c_uint, C.c_uint, C.POINTER(VolumeMode)]),
+    "lyap_calc_points": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(LyapParams), C.POINTER(C.c_int32), C.POINTER(VolumeMode)]),
+    "lyap_last_reevaluated": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "lyap_calc_render": (C.c_int, [_P, _P, _P, C.POINTER(LyapCam), C.POINTER(LyapParams), C.POINTER(C.c_int32), _P, C.c_uint, C.c_uint, C.c_uint, _P, _P, _P, C.POINTER(RenderMode)]),
+    "lyap_last_evals": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "lyap_render_batch": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning)]),
+    "lyap_autotune": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning)]),
+    "lyap_frame_capture": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning), C.POINTER(_P)]),
+    "lyap_frame_replay": (C.c_int, [_P, _P]),
+    "lyap_frame_destroy": (None, [_P]),
+    "lyap_server_create": (C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "lyap_server_destroy": (None, [_P]),
+    "lyap_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "lyap_server_progress": (C.c_int, [_P, C.c_double]),
+    "lyap_server_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(Result)]),
+    "lyap_server_release": (C.c_int, [_P, C.c_uint64]),
+    "lyap_server_cancel": (C.c_int, [_P, C.c_uint64]),
+    "lyap_server_clock_ms": (C.c_double, [_P]),
+}
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Loads liblyap_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> Sequence[str]:
+    return list(_SIGS)
+
+
+def strerror(status: int) -> str:
+    return load().lyap_strerror(status).decode()
+
+
+# -------------------------------------------------------- host helpers ----
+
+def params_init():
+    """params_init() defaults (params.cu:21-114): (prm, cam, lights, n_lights)."""
+    prm = LyapParams()
+    cam = LyapCam()
+    lights = (LyapLight * MAX_LIGHTS)()
+    n = C.c_uint()
+    load().lyap_params_init(C.byref(prm), C.byref(cam), lights, C.byref(n))
+    return prm, cam, lights, n.value
+
+
+def default_sequence() -> str:
+    return load().lyap_default_sequence().decode()
+
+
+def convert_sequence(text: str) -> list:
+    """scene_convert_sequence (scene.cu:69-108): letter indices plus -1."""
+    lib = load()
+    n = C.c_size_t()
+    rc = lib.lyap_scene_convert_sequence(text.encode(), None, 0, C.byref(n))
+    if rc == ERR_BAD_SEQUENCE:
+        raise LyapError(rc, f"bad sequence {text!r}")
+    buf = (C.c_int32 * n.value)()
+    rc = lib.lyap_scene_convert_sequence(text.encode(), buf, n.value, C.byref(n))
+    if rc != OK:
+        raise LyapError(rc, f"sequence {text!r}")
+    return list(buf)
+
+
+def seq_array(seq) -> C.Array:
+    if isinstance(seq, str):
+        seq = convert_sequence(seq)
+    seq = list(seq)
+    if not seq or seq[-1] != -1:
+        seq = seq + [-1]
+    return (C.c_int32 * len(seq))(*seq)
+
+
+def cam_recalculate(cam: LyapCam, w: int, h: int, denom: int = 1) -> LyapCam:
+    load().lyap_scene_cam_recalculate(C.byref(cam), w, h, denom)
+    return cam
+
+
+def lights_recalculate(lights, n: int):
+    load().lyap_scene_lights_recalculate(lights, n)
+    return lights
+
+
+def quat_transform(q: Quat, v) -> tuple:
+    out = Vec()
+    load().lyap_quat_transform(C.byref(q), C.byref(Vec(*v)), C.byref(out))
+    return (out.x, out.y, out.z)
+
+
+def quat_axis_angle(axis, angle: float, degrees: bool = True) -> Quat:
+    q = Quat()
+    load().lyap_quat_axis_angle(C.byref(Vec(*axis)), angle, int(degrees), C.byref(q))
+    return q
+
+
+def quat_between(p, q, scale: float = 1.0) -> Quat:
+    out = Quat()
+    load().lyap_quat_between(C.byref(Vec(*p)), C.byref(Vec(*q)), scale, C.byref(out))
+    return out
+
+
+def cam_look_at(eye, target, M: float, w: int, h: int) -> LyapCam:
+    cam = LyapCam()
+    load().lyap_cam_look_at(C.byref(cam), C.byref(Vec(*eye)), C.byref(Vec(*target)), M, w, h)
+    return cam
+
+
+def make_params(**kw) -> LyapParams:
+    """params_init() defaults with fields overridden by keyword."""
+    prm, _, _, _ = params_init()
+    for k, v in kw.items():
+        setattr(prm, k, v)
+    return prm
+
+
+def volume_mode(precision=PREC_PLAIN, renorm=16, fp64_margin=0.0, converge_tau=0.0,
+                converge_band=0.0) -> VolumeMode:
+    m = VolumeMode()
+    m.precision = precision
+    m.renorm = renorm
+    m.fp64_margin = fp64_margin
+    m.converge_tau = converge_tau
+    m.converge_band = converge_band
+    return m
+
+
+def render_mode(refine=REFINE_WALK, bisect_steps=8, normal_h=0.0, precision=PREC_PLAIN,
+                fp64_margin=0.0, lanes_per_ray=0) -> RenderMode:
+    m = RenderMode()
+    m.refine = refine
+    m.bisect_steps = bisect_steps
+    m.normal_h = normal_h
+    m.precision = precision
+    m.fp64_margin = fp64_margin
+    m.lanes_per_ray = lanes_per_ray
+    return m
+
+
+# ------------------------------------------------------------- context ----
+
+def _check(rc: int, ctx=None, what: str = ""):
+    if rc != OK:
+        msg = ""
+        if ctx is not None:
+            try:
+                msg = load().lyap_last_error(ctx).decode()
+            except Exception:
+                msg = ""
+        raise LyapError(rc, f"{what}: {strerror(rc)} {msg}".strip())
+
+
+class Context:
+    """A lyap_ctx: one device, one stream, the library's scratch."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        _check(self.lib.lyap_create(C.byref(h), device, hbm_budget), None, "lyap_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.lyap_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.lyap_get_stream(self.h) or 0)
+
+    def sync(self):
+        _check(self.lib.lyap_sync(self.h), self.h, "lyap_sync")
+
+    def bytes_in_use(self) -> int:
+        return int(self.lib.lyap_bytes_in_use(self.h))
+
+    def calc_volume(self, out_ptr: int, prm: LyapParams, seq, lo, hi, n,
+                    mode: Optional[VolumeMode] = None):
+        nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+        lo3 = (C.c_float * 3)(*lo)
+        hi3 = (C.c_float * 3)(*hi)
+        rc = self.lib.lyap_calc_volume(self.h, out_ptr, C.byref(prm), seq_array(seq), lo3, hi3,
+                                       nx, ny, nz, C.byref(mode) if mode is not None else None)
+        _check(rc, self.h, "lyap_calc_volume")
+
+    def calc_points(self, out_ptr: int, xyz_ptr: int, n: int, prm: LyapParams, seq,
+                    mode: Optional[VolumeMode] = None):
+        rc = self.lib.lyap_calc_points(self.h, out_ptr, xyz_ptr, n, C.byref(prm), seq_array(seq),
+                                       C.byref(mode) if mode is not None else None)
+        _check(rc, self.h, "lyap_calc_points")
+
+    def last_reevaluated(self) -> int:
+        c = C.c_uint64()
+        _check(self.lib.lyap_last_reevaluated(self.h, C.byref(c)), self.h, "lyap_last_reevaluated")
+        return c.value
+
+    def calc_render(self, rgba_ptr: int, points_ptr: int, cam: LyapCam, prm: LyapParams, seq,
+                    lights_ptr: int, num_lights: int, w: int, h: int, depth_ptr: int = 0,
+                    lambda_ptr: int = 0, evals_ptr: int = 0, mode: Optional[RenderMode] = None):
+        rc = self.lib.lyap_calc_render(self.h, rgba_ptr, points_ptr or None, C.byref(cam), C.byref(prm),
+                                       seq_array(seq), lights_ptr or None, num_lights, w, h,
+                                       depth_ptr or None, lambda_ptr or None, evals_ptr or None,
+                                       C.byref(mode) if mode is not None else None)
+        _check(rc, self.h, "lyap_calc_render")
+
+    def last_evals(self) -> int:
+        c = C.c_uint64()
+        _check(self.lib.lyap_last_evals(self.h, C.byref(c)), self.h, "lyap_last_evals")
+        return c.value

@@ -1,0 +1,73 @@
+/*
+ * internal.hpp - host-side state shared by the library's translation units.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <unordered_map>
+
+#include "lyap.h"
+#include "lyap_device.hpp"
+
+struct lyap_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;          /* 0: no cap */
+    size_t in_use = 0;          /* bytes the library allocated */
+    std::string last_error;
+    std::unordered_map<void *, size_t> allocations;
+
+    /* scratch: [0] FP64 re-evaluation count, [1..2] 64-bit evaluation count */
+    uint32_t *counters = nullptr;
+    uint32_t *reeval_list = nullptr;   /* voxel indices for the FP64 pass */
+    size_t reeval_cap = 0;             /* entries */
+
+    /* device view table of the batched render */
+    void *view_table = nullptr;
+    size_t view_table_bytes = 0;
+    void *view_staging = nullptr;      /* pinned host copy */
+    size_t view_staging_bytes = 0;
+    uint32_t *tile_counters = nullptr; /* work-queue heads */
+    size_t tile_counter_cap = 0;
+
+    LyapTuning tuned{};                /* autotuner's choice, zero = unset */
+    int num_cus = 0;
+};
+
+namespace lyap {
+
+int fail(lyap_ctx *ctx, int code, const char *fmt, ...);
+
+/* Allocation through the context's budget. */
+int ctx_alloc(lyap_ctx *ctx, void **ptr, size_t bytes);
+void ctx_free(lyap_ctx *ctx, void *ptr);
+
+int ensure_reeval_list(lyap_ctx *ctx, size_t entries);
+
+/* Launchers in the kernel translation units. */
+int launch_zero_u32(lyap_ctx *ctx, uint32_t *p, uint32_t n);
+
+int launch_volume(lyap_ctx *ctx, float *out, const EvalDesc &e,
+                  const float lo[3], const float hi[3], unsigned nx,
+                  unsigned ny, unsigned nz, const float *xyz, size_t npoints,
+                  const LyapVolumeMode &mode);
+
+int launch_render(lyap_ctx *ctx, LyapRGBA *rgba, LyapPoint *points,
+                  const LyapCam &cam, const LyapParams &prm, const EvalDesc &e,
+                  const LyapLight *lights, unsigned num_lights, unsigned w,
+                  unsigned h, float *depth, float *lambda, uint32_t *evals,
+                  const LyapRenderMode &mode);
+
+} /* namespace lyap */
+
+#define LYAP_HIP(ctx, call)                                                   \
+    do {                                                                      \
+        hipError_t _e = (call);                                               \
+        if (_e != hipSuccess)                                                 \
+            return lyap::fail((ctx), LYAP_ERR_DEVICE, "%s: %s (%s:%d)", #call, \
+                              hipGetErrorString(_e), __FILE__, __LINE__);     \
+    } while (0)

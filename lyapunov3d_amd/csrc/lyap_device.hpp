@@ -1,0 +1,403 @@
+/*
+ * lyap_device.hpp - the per-point Lyapunov exponent on gfx950.
+ *
+ * One lane evaluates one point. The reference routine is lyap4d
+ * (kernel.cu:108-154): x0 = 0.5, `settle` steps of x <- r x (1 - x), then
+ * `accum` steps summing ln|r_n (1 - 2 x_{n+1})| (the derivative uses the new
+ * iterate, kernel.cu:143-144); |x - 0.5| < 1e-8 after the settle returns 0
+ * (kernel.cu:138); a non-finite sum returns NaN (kernel.cu:147-149); the
+ * sequence position wraps at the -1 terminator and carries over from the
+ * settle into the accumulation (kernel.cu:133-134, 141-142).
+ *
+ * Three arithmetic forms:
+ *
+ *  plain<L>    FP32 iterate, one hardware log2 per accumulated step. The
+ *              reference method in FP32; the baseline kernel.
+ *
+ *  product<L>  FP32 iterate in u = 1 - 2x, where the map becomes
+ *              u' = (1 - r/2) + (r/2) u^2 and the derivative factor is
+ *              r_n u_{n+1}. |u| <= 1 while 0 <= r <= 4, so the running product
+ *              M of |u| only shrinks between renormalisations: it is carried
+ *              as an FP32 mantissa plus an int32 exponent, renormalised every
+ *              block of <= 16 steps, and a block that ends below 2^-100 (where
+ *              gradual underflow could have cost bits) is recomputed with a
+ *              renormalisation per step. The ln|r_n| terms depend only on the
+ *              letter, so they are added once per letter with host-computed
+ *              counts. One log2 per evaluation instead of one per step.
+ *              Per accumulated step: one multiply (u^2), one FMA, one
+ *              multiply into M.
+ *
+ *  fp64<L>     FP64 with exactly the reference's operation order
+ *              ((r*v)*(1-v) and r - (2r)*v, no fused multiply-add; the whole
+ *              library is built with -ffp-contract=off), so the orbit is
+ *              bitwise the one an IEEE double CPU computes. The log sum is
+ *              carried as a product with exponent renormalisation every 8
+ *              steps, which changes lambda only by rounding (~1e-15).
+ *
+ * Sequence handling: the period L is a template parameter, so the
+ * coefficients of each sequence position sit in VGPRs and the step loop is
+ * unrolled over one period. The letters themselves are wave-uniform (kernel
+ * arguments in SGPRs); a position's coefficient is picked from the lane's
+ * (x, y, z, d) once per evaluation.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lyap.h"
+
+namespace lyap {
+
+constexpr float kLn2f = 0.693147180559945309f;
+constexpr double kLn2 = 0.693147180559945309417232121458;
+/* A product block ending below this may have lost bits to underflow. */
+constexpr float kUnderflowGuard = 0x1p-100f;
+
+/* Wave-uniform description of one lambda evaluation: iteration counts,
+ * sequence, and host-derived loop bounds. Built by make_eval_desc(). */
+struct EvalDesc {
+    uint32_t settle, accum;
+    float d;
+    int32_t L;
+    int32_t letters[LYAP_MAX_SEQ];
+    /* settle = settle_periods * L + settle_rem */
+    uint32_t settle_periods, settle_rem;
+    /* accumulation: positions [head_begin, head_end) of the period the settle
+     * ended in, then acc_periods full periods, then positions [0, tail). */
+    uint32_t head_begin, head_end;
+    uint32_t acc_periods, tail;
+    /* first part of the window for the convergence check: the head plus
+     * half_periods full periods (half_steps steps) */
+    uint32_t half_periods, half_steps;
+    /* product form: full periods per renormalisation block */
+    uint32_t block_periods;
+    uint32_t pad;
+    /* occurrences of letters A..D in the whole window and in its first part */
+    float count[4];
+    float count_half[4];
+};
+
+/* Host: derive the loop bounds. Returns false for an unusable sequence. */
+inline bool make_eval_desc(EvalDesc *e, const LyapParams &prm,
+                           const int32_t *seq, uint32_t renorm)
+{
+    int L = 0;
+    while (L <= LYAP_MAX_SEQ && seq[L] != -1) {
+        if (seq[L] < 0 || seq[L] > 3) return false;
+        ++L;
+    }
+    if (L == 0 || L > LYAP_MAX_SEQ) return false;
+    *e = EvalDesc{};
+    e->settle = prm.settle;
+    e->accum = prm.accum;
+    e->d = prm.d;
+    e->L = L;
+    for (int j = 0; j < L; ++j) e->letters[j] = seq[j];
+    e->settle_periods = prm.settle / L;
+    e->settle_rem = prm.settle % L;
+    uint32_t left = prm.accum;
+    e->head_begin = e->settle_rem;
+    e->head_end = e->settle_rem;
+    if (e->settle_rem != 0) {
+        const uint32_t h = (uint32_t)L - e->settle_rem;
+        const uint32_t take = left < h ? left : h;
+        e->head_end = e->settle_rem + take;
+        left -= take;
+    }
+    e->acc_periods = left / L;
+    e->tail = left % L;
+    e->half_periods = e->acc_periods / 2;
+    e->half_steps = (e->head_end - e->head_begin) + e->half_periods * (uint32_t)L;
+    if (renorm == 0) renorm = 16;
+    uint32_t bp = renorm / (uint32_t)L;
+    e->block_periods = bp ? bp : 1;
+    /* letter counts */
+    for (int k = 0; k < 4; ++k) { e->count[k] = 0.f; e->count_half[k] = 0.f; }
+    for (uint32_t j = e->head_begin; j < e->head_end; ++j) {
+        e->count[seq[j]] += 1.f;
+        e->count_half[seq[j]] += 1.f;
+    }
+    for (int j = 0; j < L; ++j) {
+        e->count[seq[j]] += (float)e->acc_periods;
+        e->count_half[seq[j]] += (float)e->half_periods;
+    }
+    for (uint32_t j = 0; j < e->tail; ++j) e->count[seq[j]] += 1.f;
+    return true;
+}
+
+/* Lane's value of sequence letter `s` (0..3 = x, y, z, d); s is uniform. */
+__device__ __forceinline__ float pick(int s, float x, float y, float z, float d)
+{
+    return s == 0 ? x : (s == 1 ? y : (s == 2 ? z : d));
+}
+__device__ __forceinline__ double pickd(int s, double x, double y, double z,
+                                        double d)
+{
+    return s == 0 ? x : (s == 1 ? y : (s == 2 ? z : d));
+}
+
+/* ---- plain FP32: one log2 per accumulated step ------------------------- */
+
+template <int L>
+__device__ __forceinline__ float lambda_plain(float x, float y, float z,
+                                              const EvalDesc &e)
+{
+    float r[L], m2r[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        r[j] = pick(e.letters[j], x, y, z, e.d);
+        m2r[j] = -2.0f * r[j];
+    }
+    float v = 0.5f;
+    for (uint32_t p = 0; p < e.settle_periods; ++p) {
+#pragma unroll
+        for (int j = 0; j < L; ++j) v = r[j] * __builtin_fmaf(-v, v, v);
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+        if ((uint32_t)j < e.settle_rem) v = r[j] * __builtin_fmaf(-v, v, v);
+
+    if (v == 0.5f) return 0.0f / (float)e.accum;   /* kernel.cu:138 in FP32 */
+
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        if ((uint32_t)j >= e.head_begin && (uint32_t)j < e.head_end) {
+            v = r[j] * __builtin_fmaf(-v, v, v);
+            s += __builtin_amdgcn_logf(__builtin_fabsf(__builtin_fmaf(m2r[j], v, r[j])));
+        }
+    }
+    for (uint32_t p = 0; p < e.acc_periods; ++p) {
+#pragma unroll
+        for (int j = 0; j < L; ++j) {
+            v = r[j] * __builtin_fmaf(-v, v, v);
+            s += __builtin_amdgcn_logf(__builtin_fabsf(__builtin_fmaf(m2r[j], v, r[j])));
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        if ((uint32_t)j < e.tail) {
+            v = r[j] * __builtin_fmaf(-v, v, v);
+            s += __builtin_amdgcn_logf(__builtin_fabsf(__builtin_fmaf(m2r[j], v, r[j])));
+        }
+    }
+    const float lam = s * kLn2f / (float)e.accum;
+    return __builtin_isfinite(lam) ? lam : __builtin_nanf("");
+}
+
+/* ---- FP32 u-form with a mantissa/exponent product ---------------------- */
+
+struct ProductState {
+    float M;     /* mantissa of the running product of |u| */
+    int E;       /* its binary exponent */
+};
+
+__device__ __forceinline__ void renorm(ProductState &st)
+{
+    st.E += __builtin_amdgcn_frexp_expf(st.M);
+    st.M = __builtin_amdgcn_frexp_mantf(st.M);
+}
+
+/* ln of the product plus the letter terms, divided by n; NaN if not finite. */
+__device__ __forceinline__ float finish_product(const ProductState &st,
+                                                const float lg[4],
+                                                const float cnt[4], uint32_t n)
+{
+    double t = (double)st.E + (double)__builtin_amdgcn_logf(st.M);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (cnt[k] != 0.0f) t += (double)cnt[k] * (double)lg[k];
+    const float lam = (float)(t * kLn2 / (double)n);
+    return __builtin_isfinite(lam) ? lam : __builtin_nanf("");
+}
+
+/* Result of a product-form evaluation. half1/half2 are the lambdas of the
+ * two parts of the window (NaN when a part is empty). */
+struct ProductResult {
+    float lam;
+    float half1, half2;
+};
+
+/* `n` full periods in blocks of `bp` periods, renormalising after each block
+ * and recomputing a block whose product fell below kUnderflowGuard. */
+template <int L>
+__device__ __forceinline__ void product_periods(float &u, ProductState &st,
+                                                const float (&a)[L],
+                                                const float (&b)[L],
+                                                uint32_t n, uint32_t bp)
+{
+    for (uint32_t p = 0; p < n; p += bp) {
+        const uint32_t cnt = n - p < bp ? n - p : bp;
+        const float u0 = u;
+        const ProductState s0 = st;
+        for (uint32_t k = 0; k < cnt; ++k) {
+#pragma unroll
+            for (int j = 0; j < L; ++j) {
+                u = __builtin_fmaf(b[j], u * u, a[j]);
+                st.M *= __builtin_fabsf(u);
+            }
+        }
+        if (st.M < kUnderflowGuard) {
+            float w = u0;
+            st = s0;
+            for (uint32_t k = 0; k < cnt; ++k) {
+#pragma unroll
+                for (int j = 0; j < L; ++j) {
+                    w = __builtin_fmaf(b[j], w * w, a[j]);
+                    st.M *= __builtin_fabsf(w);
+                    renorm(st);
+                }
+            }
+        }
+        renorm(st);
+    }
+}
+
+template <int L, bool HALVES>
+__device__ __forceinline__ ProductResult lambda_product(float x, float y,
+                                                        float z,
+                                                        const EvalDesc &e)
+{
+    float a[L], b[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        b[j] = 0.5f * pick(e.letters[j], x, y, z, e.d);
+        a[j] = 1.0f - b[j];
+    }
+    float u = 0.0f;
+    for (uint32_t p = 0; p < e.settle_periods; ++p) {
+#pragma unroll
+        for (int j = 0; j < L; ++j) u = __builtin_fmaf(b[j], u * u, a[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+        if ((uint32_t)j < e.settle_rem) u = __builtin_fmaf(b[j], u * u, a[j]);
+
+    ProductResult res;
+    /* kernel.cu:138: |x - 0.5| < 1e-8 is |u| < 2e-8 */
+    if (__builtin_fabsf(u) < 2e-8f) {
+        res.lam = 0.0f / (float)e.accum;
+        res.half1 = res.half2 = res.lam;
+        return res;
+    }
+
+    ProductState st{1.0f, 0};
+    /* head: the rest of the period the settle ended in, renormalised per step */
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        if ((uint32_t)j >= e.head_begin && (uint32_t)j < e.head_end) {
+            u = __builtin_fmaf(b[j], u * u, a[j]);
+            st.M *= __builtin_fabsf(u);
+            renorm(st);
+        }
+    }
+    ProductState st_half{1.0f, 0};
+    if (HALVES) {
+        product_periods<L>(u, st, a, b, e.half_periods, e.block_periods);
+        st_half = st;
+        product_periods<L>(u, st, a, b, e.acc_periods - e.half_periods,
+                           e.block_periods);
+    } else {
+        product_periods<L>(u, st, a, b, e.acc_periods, e.block_periods);
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        if ((uint32_t)j < e.tail) {
+            u = __builtin_fmaf(b[j], u * u, a[j]);
+            st.M *= __builtin_fabsf(u);
+            renorm(st);
+        }
+    }
+
+    float lg[4];
+    lg[0] = __builtin_amdgcn_logf(__builtin_fabsf(x));
+    lg[1] = __builtin_amdgcn_logf(__builtin_fabsf(y));
+    lg[2] = __builtin_amdgcn_logf(__builtin_fabsf(z));
+    lg[3] = __builtin_amdgcn_logf(__builtin_fabsf(e.d));
+    res.lam = finish_product(st, lg, e.count, e.accum);
+    res.half1 = res.half2 = res.lam;
+    if (HALVES) {
+        const uint32_t n1 = e.half_steps, n2 = e.accum - e.half_steps;
+        float c2[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c2[k] = e.count[k] - e.count_half[k];
+        ProductState second;
+        second.E = st.E - st_half.E;
+        second.M = st.M / st_half.M;
+        res.half1 = n1 ? finish_product(st_half, lg, e.count_half, n1) : __builtin_nanf("");
+        res.half2 = n2 ? finish_product(second, lg, c2, n2) : __builtin_nanf("");
+    }
+    return res;
+}
+
+/* ---- FP64 with the reference's operation order ------------------------- */
+
+struct ProductState64 {
+    double M;
+    int E;
+};
+
+__device__ __forceinline__ void renorm64(ProductState64 &st)
+{
+    st.E += __builtin_amdgcn_frexp_exp(st.M);
+    st.M = __builtin_amdgcn_frexp_mant(st.M);
+}
+
+template <int L>
+__device__ __forceinline__ double lambda_fp64(float xf, float yf, float zf,
+                                              const EvalDesc &e)
+{
+    const double x = xf, y = yf, z = zf, d = e.d;
+    double r[L], r2[L];
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        r[j] = pickd(e.letters[j], x, y, z, d);
+        r2[j] = 2.0 * r[j];          /* exact */
+    }
+    double v = 0.5;
+    for (uint32_t p = 0; p < e.settle_periods; ++p) {
+#pragma unroll
+        for (int j = 0; j < L; ++j) v = (r[j] * v) * (1.0 - v);
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+        if ((uint32_t)j < e.settle_rem) v = (r[j] * v) * (1.0 - v);
+
+    const double off = v - 0.5;
+    if (!(off <= -1e-8 || off >= 1e-8)) return 0.0 / (double)e.accum;
+
+    ProductState64 st{1.0, 0};
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        if ((uint32_t)j >= e.head_begin && (uint32_t)j < e.head_end) {
+            v = (r[j] * v) * (1.0 - v);
+            st.M *= __builtin_fabs(r[j] - r2[j] * v);
+            renorm64(st);
+        }
+    }
+    /* factors are at most max|r| * max|1 - 2v|; 8 of them cannot leave the
+     * double range for coordinates of float magnitude (or the orbit escapes,
+     * which makes lambda NaN either way) */
+    for (uint32_t p = 0; p < e.acc_periods; ++p) {
+#pragma unroll
+        for (int j = 0; j < L; ++j) {
+            v = (r[j] * v) * (1.0 - v);
+            st.M *= __builtin_fabs(r[j] - r2[j] * v);
+            if (((j + 1) & 7) == 0) renorm64(st);
+        }
+        if ((L & 7) != 0) renorm64(st);
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        if ((uint32_t)j < e.tail) {
+            v = (r[j] * v) * (1.0 - v);
+            st.M *= __builtin_fabs(r[j] - r2[j] * v);
+            renorm64(st);
+        }
+    }
+    const double lam = ((double)st.E * kLn2 + log(st.M)) / (double)e.accum;
+    return __builtin_isfinite(lam) ? lam : __builtin_nan("");
+}
+
+} /* namespace lyap */

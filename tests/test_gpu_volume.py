@@ -1,0 +1,170 @@
+"""Volume and point lambda on the GPU against the oracle.
+
+Parity bar (floating point, lambda compared rather than chaotic iterates):
+ - LYAP_PREC_FP64 reproduces the oracle's FP64 orbit, so every voxel agrees
+   to 1e-9 and NaN positions coincide;
+ - FP32 forms: on stable voxels (FP64 lambda < -1e-3 with the two halves of
+   the window agreeing to 1e-4, i.e. the orbit has settled) |dlambda| <= 1e-3
+   on >= 99% (plain) / 99.9% (product with FP64 re-evaluation); lambda sign
+   agreement >= 99% (plain) / 99.9% (re-evaluated).
+FP32 rounding can prolong a chaotic transient or select a coexisting
+attractor; the reference's own FP32 build shows the same effect.
+"""
+import numpy as np
+import pytest
+
+import lyapunov3d_amd as lyap
+from oracle import oracle as O
+from tests import workloads as W
+from tests.gpu_util import empty, to_numpy, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    torch_cuda()
+    c = lyap.Context(0)
+    yield c
+    c.close()
+
+
+def gpu_volume(ctx, prm, seq, lo, hi, n, mode=None):
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+    out = empty(4 * nx * ny * nz)
+    ctx.calc_volume(out.data_ptr(), prm, seq, lo, hi, (nx, ny, nz), mode)
+    ctx.sync()
+    return to_numpy(out, np.float32, (nz, ny, nx))
+
+
+def stats(got, ref, h1, h2):
+    stable = (ref < -1e-3) & (np.abs(h1 - h2) < 1e-4)
+    d = np.abs(got[stable].astype(np.float64) - ref[stable])
+    within = float(np.mean(np.nan_to_num(d, nan=9.0) <= 1e-3)) if d.size else 1.0
+    sign = np.mean((np.sign(got) == np.sign(ref)) | (np.isnan(got) & np.isnan(ref)))
+    return dict(stable=float(stable.mean()), within=within, sign=float(sign),
+                max=float(np.nanmax(d)) if d.size else 0.0,
+                rms=float(np.sqrt(np.nanmean(d ** 2))) if d.size else 0.0)
+
+
+def c1_reference():
+    prm, seq, lo, hi, n = W.volume_config("c1")
+    ref = O.volume(prm, seq, lo, hi, n, mode=O.F64).astype(np.float64)
+    c = O.grid_coords(lo[0], hi[0], n)
+    zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
+    xyz = np.stack([xx, yy, zz], -1).reshape(-1, 3)
+    h1, h2 = O.lyap_halves(xyz, prm.d, prm.settle, prm.accum, prm.accum // 2, seq)
+    return ref, h1.reshape(ref.shape), h2.reshape(ref.shape)
+
+
+@pytest.fixture(scope="module")
+def c1_ref():
+    return c1_reference()
+
+
+def test_c1_fp64_matches_oracle_exactly(ctx, c1_ref):
+    prm, seq, lo, hi, n = W.volume_config("c1")
+    ref = O.volume(prm, seq, lo, hi, n, mode=O.F64)   # rounded to float like the GPU output
+    got = gpu_volume(ctx, prm, seq, lo, hi, n, lyap.volume_mode(lyap.PREC_FP64))
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    m = ~np.isnan(ref)
+    assert np.max(np.abs(got[m].astype(np.float64) - ref[m])) <= 1e-6
+
+
+def test_c1_plain(ctx, c1_ref):
+    ref, h1, h2 = c1_ref
+    prm, seq, lo, hi, n = W.volume_config("c1")
+    got = gpu_volume(ctx, prm, seq, lo, hi, n)
+    s = stats(got, ref, h1, h2)
+    print("c1 plain", s)
+    assert s["within"] >= 0.99 and s["sign"] >= 0.99
+
+
+def test_c1_product(ctx, c1_ref):
+    ref, h1, h2 = c1_ref
+    prm, seq, lo, hi, n = W.volume_config("c1")
+    got = gpu_volume(ctx, prm, seq, lo, hi, n, lyap.volume_mode(lyap.PREC_PRODUCT))
+    s = stats(got, ref, h1, h2)
+    print("c1 product", s)
+    assert s["within"] >= 0.99 and s["sign"] >= 0.99
+
+
+def test_c1_product_reevaluated(ctx, c1_ref):
+    ref, h1, h2 = c1_ref
+    prm, seq, lo, hi, n = W.volume_config("c1")
+    got = gpu_volume(ctx, prm, seq, lo, hi, n, lyap.volume_mode(**W.C3_STRICT))
+    nre = ctx.last_reevaluated()
+    s = stats(got, ref, h1, h2)
+    print("c1 product+fp64", s, "reevaluated", nre / got.size)
+    assert s["within"] >= 0.999 and s["sign"] >= 0.999
+
+
+def test_c3_sampled(ctx):
+    """Full 256^3 precision-form volume; the oracle checks 20k seeded voxels."""
+    prm, seq, lo, hi, n = W.volume_config("c3")
+    got = gpu_volume(ctx, prm, seq, lo, hi, n, lyap.volume_mode(**W.C3_STRICT))
+    rng = np.random.default_rng(3)
+    idx = rng.integers(0, n, size=(20000, 3))
+    c = O.grid_coords(lo[0], hi[0], n)
+    xyz = np.stack([c[idx[:, 0]], c[idx[:, 1]], c[idx[:, 2]]], -1)
+    ref = O.lyap_points(xyz, prm.d, prm.settle, prm.accum, seq)
+    h1, h2 = O.lyap_halves(xyz, prm.d, prm.settle, prm.accum, prm.accum // 2, seq)
+    g = got[idx[:, 2], idx[:, 1], idx[:, 0]]
+    s = stats(g, ref, h1, h2)
+    print("c3 product+fp64", s)
+    assert s["within"] >= 0.999 and s["sign"] >= 0.999
+
+
+@pytest.mark.parametrize("prec", [lyap.PREC_PLAIN, lyap.PREC_PRODUCT, lyap.PREC_FP64])
+def test_special_points(ctx, prec):
+    """Closed forms on the diagonal and the reference's special cases."""
+    torch = torch_cuda()
+    pts = np.array([[4, 4, 4], [2, 2, 2], [2.5, 2.5, 2.5], [3.2, 3.2, 3.2], [2.9, 2.9, 2.9]],
+                   dtype=np.float32)
+    xyz = torch.from_numpy(pts.reshape(-1)).cuda()
+    out = empty(4 * len(pts))
+    prm = lyap.make_params(settle=200, accum=800)
+    ctx.calc_points(out.data_ptr(), xyz.data_ptr(), len(pts), prm, "A", lyap.volume_mode(prec))
+    ctx.sync()
+    got = to_numpy(out, np.float32)
+    r = pts[:, 0].astype(np.float64)
+    expect = np.array([np.log(4.0), 0.0, np.log(abs(2 - r[2])), 0.5 * np.log(abs(4 + 2 * r[3] - r[3] ** 2)),
+                       np.log(abs(2 - r[4]))])
+    tol = 1e-5 if prec != lyap.PREC_PLAIN else 1e-4
+    assert np.allclose(got, expect, atol=tol, rtol=0), (got, expect)
+
+
+@pytest.mark.parametrize("prec", [lyap.PREC_PLAIN, lyap.PREC_PRODUCT, lyap.PREC_FP64])
+def test_edge_cases(ctx, prec):
+    """settle 0 (0.5 guard), accum 0 (0/0), ragged sizes, a period-32 sequence,
+    the D letter, escaping orbits (r > 4) giving NaN."""
+    mode = lyap.volume_mode(prec)
+    g = gpu_volume(ctx, lyap.make_params(settle=0, accum=50), "AB", (2, 2, 2), (4, 4, 4), (5, 3, 2), mode)
+    assert np.all(g == 0.0)
+    g = gpu_volume(ctx, lyap.make_params(settle=10, accum=0), "AB", (2, 2, 2), (4, 4, 4), (5, 3, 2), mode)
+    assert np.all(np.isnan(g))
+    seq32 = "ABCD" * 8
+    prm = lyap.make_params(settle=37, accum=101, d=3.3)
+    for seq, n in [("A6B6C6D6", (17, 5, 3)), (seq32, (9, 7, 5)), ("DCBA", (13, 1, 1))]:
+        g = gpu_volume(ctx, prm, seq, (2.1, 2.2, 2.3), (3.9, 4.0, 3.7), n, mode)
+        ref = O.volume(prm, seq, (2.1, 2.2, 2.3), (3.9, 4.0, 3.7), n, mode=O.F64)
+        assert np.array_equal(np.isnan(g), np.isnan(ref)), seq
+        m = ~np.isnan(ref) & (ref < -0.05)
+        tol = 1e-6 if prec == lyap.PREC_FP64 else 5e-3
+        assert np.mean(np.abs(g[m] - ref[m]) <= tol) >= 0.95, seq
+    # r > 4 escapes: NaN like the reference
+    g = gpu_volume(ctx, lyap.make_params(settle=20, accum=20), "AB", (4.2, 4.2, 4.2), (5, 5, 5), (4, 4, 4), mode)
+    assert np.all(np.isnan(g))
+
+
+def test_errors(ctx):
+    out = empty(64)
+    prm = lyap.make_params()
+    with pytest.raises(lyap.LyapError) as e:
+        ctx.calc_volume(out.data_ptr(), prm, [0] * 33 + [-1], (0, 0, 0), (4, 4, 4), 2)
+    assert e.value.status == lyap.ERR_BAD_SEQUENCE
+    with pytest.raises(lyap.LyapError) as e:
+        ctx.calc_volume(out.data_ptr(), prm, [0, 5, -1], (0, 0, 0), (4, 4, 4), 2)
+    assert e.value.status == lyap.ERR_BAD_SEQUENCE
+    with pytest.raises(lyap.LyapError):
+        ctx.calc_volume(out.data_ptr(), prm, "AB", (0, 0, 0), (4, 4, 4), 2, lyap.volume_mode(precision=7))

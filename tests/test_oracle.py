@@ -1,0 +1,242 @@
+"""Pins the CPU oracle (oracle/lyap_oracle.cpp) without trusting its formula.
+
+The reference ships no tests or golden data for this path, and its sources
+need CUDA's vector types and runtime headers, so it cannot be built here
+(see the oracle's header). Each check below comes from a route other than
+the oracle's own loop, chosen so that a plausible slip fails one of them:
+
+ - exact hand cases with dyadic iterates (worked out by hand in the
+   docstrings): pin the derivative pairing r_n(1 - 2 x_{n+1})
+   (kernel.cu:143-144), the sequence position carrying over from settle to
+   accumulation (kernel.cu:133,141), the wrap at the terminator, the mean;
+ - closed forms on the diagonal x = y = z = r: fixed point ln|2 - r|,
+   2-cycle (1/2) ln|4 + 2r - r^2|, corner r = 4 gives ln 4, r = 2 gives 0
+   through the 0.5 guard (kernel.cu:138);
+ - special cases: settle 0 -> 0, accum 0 -> NaN (0/0), r = 0 and r > 4 -> NaN
+   (kernel.cu:147-149);
+ - invariant: swapping x <-> y together with A <-> B changes nothing;
+ - brute force: 60-digit mpmath iteration on stable points;
+ - raymarch: the one-letter analytic scene (surface plane at x = 2 + e^-0.5)
+   and the slab clip / pixel-corner ray in closed form;
+ - shade: Blinn-Phong with camera, light and normal aligned, in closed form.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import lyapunov3d_amd as lyap
+from oracle import oracle as O
+
+
+def lam64(p, seq, settle, accum, d=2.1):
+    return float(O.lyap_points(np.array([p], np.float32), d, settle, accum, seq)[0])
+
+
+# ---------------------------------------------------------------- hand cases
+
+def test_pairing_one_step():
+    """x = 3, y = 2.5, sequence AB, settle 1, accum 1.
+    settle (A, r=3):   x1 = 3 * 1/2 * 1/2 = 3/4
+    accum  (B, r=2.5): x2 = 5/2 * 3/4 * 1/4 = 15/32
+                       term = |5/2 * (1 - 2*15/32)| = 5/32
+    lambda = ln(5/32). Pairing r with the old iterate would give
+    ln|5/2 (1 - 3/2)| = ln 5/4; restarting the sequence for the
+    accumulation would use r = 3 again (ln 3/8)."""
+    got = lam64((3.0, 2.5, 3.5), "AB", 1, 1)
+    assert got == pytest.approx(math.log(5 / 32), abs=1e-15)
+    assert abs(got - math.log(5 / 4)) > 0.1 and abs(got - math.log(3 / 8)) > 0.1
+
+
+def test_two_steps_each():
+    """AB, settle 2, accum 2: x1 = 3/4 (A), x2 = 15/32 (B);
+    accum A: x3 = 3 * 15/32 * 17/32 = 765/1024, term |3(1 - 1530/1024)| = 759/512;
+    accum B: x4 = 5/2 * 765/1024 * 259/1024 = 990675/2^21,
+             term |5/2 (1 - 990675/2^20)| = 289505/2^21.
+    lambda = (ln 759/512 + ln 289505/2^21) / 2."""
+    expect = (math.log(759 / 512) + math.log(289505 / 2 ** 21)) / 2
+    assert lam64((3.0, 2.5, 3.5), "AB", 2, 2) == pytest.approx(expect, abs=1e-15)
+
+
+def test_three_letter_wrap():
+    """ABC with x=3, y=2.5, z=3.5, settle 2 (A, B), accum 3 (C, A, B):
+    terms 5327/2048, 2066361/2^21, 10129544271295/2^45 (exact dyadics)."""
+    expect = (math.log(5327 / 2048) + math.log(2066361 / 2 ** 21) +
+              math.log(10129544271295 / 2 ** 45)) / 3
+    assert lam64((3.0, 2.5, 3.5), "ABC", 2, 3) == pytest.approx(expect, abs=1e-14)
+
+
+# -------------------------------------------------------------- closed forms
+
+@pytest.mark.parametrize("r", [2.25, 2.5, 2.75, 2.9])
+def test_fixed_point(r):
+    """Constant r in (1,3): attracting fixed point 1 - 1/r, derivative 2 - r
+    (r taken at its float32 value, the coordinate the path sees)."""
+    r32 = float(np.float32(r))
+    assert lam64((r32, r32, r32), "ABC", 300, 1000) == pytest.approx(math.log(abs(2 - r32)), abs=1e-9)
+
+
+@pytest.mark.parametrize("r", [3.1, 3.2, 3.3, 3.4])
+def test_two_cycle(r):
+    """Constant r in (3, 1+sqrt 6): 2-cycle with multiplier 4 + 2r - r^2."""
+    r32 = float(np.float32(r))
+    got = lam64((r32, r32, r32), "A", 400, 1000)
+    assert got == pytest.approx(0.5 * math.log(abs(4 + 2 * r32 - r32 * r32)), abs=1e-9)
+
+
+def test_corners():
+    """r = 4: 1/2 -> 1 -> 0 -> 0..., every term |4(1 - 0)| = 4: ln 4 exactly.
+    r = 2: 1/2 stays 1/2, so the guard after the settle returns 0."""
+    assert lam64((4, 4, 4), "ABBAC", 100, 400) == pytest.approx(math.log(4.0), abs=1e-14)
+    assert lam64((2, 2, 2), "ABBAC", 100, 400) == 0.0
+
+
+def test_special_cases():
+    assert lam64((3.3, 2.7, 3.9), "ABC", 0, 50) == 0.0                    # guard at x0
+    assert math.isnan(lam64((3.3, 2.7, 3.9), "ABC", 10, 0))               # 0/0
+    assert math.isnan(lam64((0.0, 3.0, 3.0), "AB", 5, 10))                # r = 0 -> log 0
+    assert math.isnan(lam64((4.5, 3.0, 3.0), "AB", 5, 50))                # escapes
+    assert lam64((4, 4, 4), "D", 10, 10, d=4.0) == math.log(4.0)          # D letter is prm.d
+
+
+def test_swap_invariance():
+    rng = np.random.default_rng(0)
+    p = rng.uniform(2.0, 4.0, (500, 3)).astype(np.float32)
+    a = O.lyap_points(p, 2.1, 50, 200, "AABCB")
+    b = O.lyap_points(p[:, [1, 0, 2]], 2.1, 50, 200, "BBACA")
+    assert np.array_equal(a, b, equal_nan=True)
+
+
+def test_mpmath_brute_force():
+    """60-digit iteration on points whose orbit is attracted (lambda < -0.1)."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 60
+    rng = np.random.default_rng(5)
+    p = rng.uniform(2.0, 4.0, (400, 3)).astype(np.float32)
+    seq = [0, 1, 1, 0, 2]
+    f = O.lyap_points(p, 2.1, 60, 120, "ABBAC")
+    chosen = np.nonzero(f < -0.1)[0][:12]
+    assert len(chosen) >= 6
+    for i in chosen:
+        c = [mp.mpf(float(v)) for v in p[i]]
+        v = mp.mpf(1) / 2
+        pos = 0
+        for _ in range(60):
+            r = c[seq[pos]]; pos = (pos + 1) % 5; v = r * v * (1 - v)
+        s = mp.mpf(0)
+        for _ in range(120):
+            r = c[seq[pos]]; pos = (pos + 1) % 5; v = r * v * (1 - v)
+            s += mp.log(abs(r * (1 - 2 * v)))
+        assert float(s / 120) == pytest.approx(f[i], abs=1e-10)
+
+
+def test_fp32_as_written_close_on_settled_points():
+    rng = np.random.default_rng(2)
+    p = rng.uniform(2.0, 4.0, (3000, 3)).astype(np.float32)
+    a = O.lyap_points(p, 2.1, 100, 400, "ABBAC", mode=O.F64)
+    h1, h2 = O.lyap_halves(p, 2.1, 100, 400, 200, "ABBAC")
+    b = O.lyap_points(p, 2.1, 100, 400, "ABBAC", mode=O.F32W)
+    settled = (a < -0.05) & (np.abs(h1 - h2) < 1e-5)
+    assert settled.sum() > 500
+    assert np.mean(np.abs(a[settled] - b[settled]) < 1e-3) > 0.99
+
+
+def test_volume_coordinates_and_layout():
+    """Index x + (y + z*ny)*nx and coordinate lo + (hi-lo)*i/n (kernel.cu:525-529)."""
+    prm = lyap.make_params(settle=20, accum=40)
+    vol = O.volume(prm, "ABC", (2.0, 2.5, 3.0), (4.0, 3.5, 3.9), (5, 4, 3))
+    assert vol.shape == (3, 4, 5)
+    f = np.float32
+    lo, hi, n = (f(2.0), f(2.5), f(3.0)), (f(4.0), f(3.5), f(3.9)), (f(5), f(4), f(3))
+    for (z, y, x) in [(0, 0, 0), (2, 3, 4), (1, 2, 0)]:
+        p = [lo[k] + ((hi[k] - lo[k]) * f(i)) / n[k] for k, i in enumerate((x, y, z))]
+        assert vol[z, y, x] == pytest.approx(lam64(p, "ABC", 20, 40), abs=1e-6)
+    assert O.coord(0.0, 4.0, 3, 8) == 1.5
+
+
+# ------------------------------------------------------------------ raymarch
+
+def analytic_scene(w=24, h=24, refine=lyap.REFINE_BISECT):
+    prm, _, lights, n = lyap.params_init()
+    prm.settle, prm.accum = 200, 200
+    prm.lMin, prm.lMax = 2.0, 3.0
+    prm.opaqueThreshold = -0.5
+    prm.chaosThreshold = 0.5
+    prm.stepMethod = 1
+    prm.depth = 300.0
+    prm.jitter = 0.0
+    prm.nearMultiplier = 1.0
+    cam = lyap.cam_look_at((4.0, 2.5, 2.5), (2.0, 2.5, 2.5), 0.2, w, h)
+    lyap.lights_recalculate(lights, n)
+    mode = lyap.render_mode(refine=refine, bisect_steps=8, normal_h=1e-3 if refine else 0.0)
+    return cam, prm, lights, n, mode
+
+
+@pytest.mark.parametrize("refine", [lyap.REFINE_BISECT, lyap.REFINE_WALK])
+def test_analytic_plane(refine):
+    """Sequence 'A' in [2,3]^3: lambda = ln|2 - x|; opaque (<= -0.5) for
+    x <= 2 + e^-0.5, so the surface seen from +x is that plane, normal +x."""
+    w = h = 24
+    cam, prm, lights, n, mode = analytic_scene(w, h, refine)
+    out = O.render(cam, prm, "A", lights, n, w, h, mode=mode)
+    x_plane = 2.0 + math.exp(-0.5)
+    P = out["points"][..., :3].reshape(-1, 3)
+    N = out["points"][..., 3:6].reshape(-1, 3)
+    assert not np.any(np.isnan(out["depth"]))
+    for i in range(w * h):
+        y, x = divmod(i, w)
+        _, V, t0, t1 = O.ray_setup(cam, prm, x, y)
+        step = (t1 - t0) / prm.depth * abs(V[0])
+        assert abs(P[i, 0] - x_plane) <= step * 1.001 + 1e-6
+    assert np.all(N[:, 0] > 0.999)
+    # bisection ends within step/256 of the crossing
+    if refine == lyap.REFINE_BISECT:
+        assert np.max(np.abs(P[:, 0] - x_plane)) < 2e-4
+
+
+def test_slab_clip_closed_form():
+    """Camera on the x axis of [2,3]^3 at x=4: the centre ray enters at x=3
+    (t = 1/|V|... with |V| = 1/M) and leaves at x = 2."""
+    w = h = 2
+    cam, prm, lights, n, mode = analytic_scene(w, h)
+    M = cam.M
+    hit, V, t0, t1 = O.ray_setup(cam, prm, 1, 1)     # pixel corner (1,1) = centre ray
+    assert hit
+    assert np.allclose(V, [-1.0 / M, 0.0, 0.0], atol=1e-6)
+    assert t0 == pytest.approx(1.0 * M, rel=1e-6)
+    assert t1 == pytest.approx(2.0 * M, rel=1e-6)
+    hit, V, t0, t1 = O.ray_setup(cam, prm, 0, 0)     # corner ray: S0 direction
+    S0 = np.array([cam.S0.x, cam.S0.y, cam.S0.z])
+    assert np.allclose(V, S0 / np.linalg.norm(S0) / M, atol=1e-6)
+
+
+def test_miss_record():
+    cam, prm, lights, n, mode = analytic_scene(4, 4)
+    cam = lyap.cam_look_at((4.0, 9.0, 9.0), (4.0, 12.0, 12.0), 0.2, 4, 4)   # looks away
+    out = O.render(cam, prm, "A", lights, n, 4, 4, mode=mode)
+    assert np.all(np.isnan(out["depth"])) and np.all(out["rgba"] == 0)
+    assert np.all(np.isnan(out["points"][..., 6])) and np.all(np.isnan(out["points"][..., 8]))
+
+
+# --------------------------------------------------------------------- shade
+
+def test_shade_closed_form():
+    """Point at the origin, normal +z, camera and light on +z: L.N = 1,
+    N.H = 1, so colour = (Cs*ks + Cd*kd) * range / d^2 + ambient."""
+    prm, cam, lights, n = lyap.params_init()
+    L = lights[0]
+    L.C = lyap.Vec(0.0, 0.0, 2.0)
+    L.V = lyap.Vec(0.0, 0.0, -1.0)       # spot axis pointing at the point
+    L.lightOuterCone, L.lightInnerCone = 0.5, 0.9
+    cam.C = lyap.Vec(0.0, 0.0, 5.0)
+    pt = np.zeros(9, np.float32)
+    pt[5] = 1.0                            # N = +z
+    col, rgba = O.shade(pt, cam, lights, 1)
+    k = L.lightRange / 4.0
+    expect = [(L.specularColor.x * L.specularPower + L.diffuseColor.x * L.diffusePower) * k + L.ambient.x,
+              (L.specularColor.y * L.specularPower + L.diffuseColor.y * L.diffusePower) * k + L.ambient.y]
+    assert col[0, :2] == pytest.approx(expect, rel=1e-6)
+    assert rgba[0, 0] == 255                 # saturated (the reference's cast is undefined there)
+    pt[6] = np.nan                           # miss -> transparent black
+    col, rgba = O.shade(pt, cam, lights, 1)
+    assert np.all(col == 0) and np.all(rgba == 0)
