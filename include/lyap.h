@@ -209,6 +209,11 @@ int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_points,
                      float *d_depth, float *d_lambda, uint32_t *d_evals,
                      const LyapRenderMode *mode);
 
+/* Diagnostic: a device buffer of 4 uint64 per wave of the persistent render
+ * grid that later render launches fill with {rounds, rays started, start
+ * clock, end clock}; NULL switches it off. No reference counterpart. */
+int lyap_debug_wave_log(lyap_ctx *ctx, void *d_log);
+
 /* Total algorithmic lambda evaluations of the last render/batch call
  * (march samples through the first opaque one, refinement, normal). Waits
  * for that call to finish. */

@@ -211,6 +211,7 @@ _SIGS = {
     "lyap_last_reevaluated": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "lyap_calc_render": (C.c_int, [_P, _P, _P, C.POINTER(LyapCam), C.POINTER(LyapParams), C.POINTER(C.c_int32), _P, C.c_uint, C.c_uint, C.c_uint, _P, _P, _P, C.POINTER(RenderMode)]),
     "lyap_last_evals": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "lyap_debug_wave_log": (C.c_int, [_P, _P]),
     "lyap_render_batch": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning)]),
     "lyap_autotune": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning)]),
     "lyap_frame_capture": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning), C.POINTER(_P)]),

@@ -31,10 +31,13 @@ struct lyap_ctx {
     size_t view_table_bytes = 0;
     void *view_staging = nullptr;      /* pinned host copy */
     size_t view_staging_bytes = 0;
-    uint32_t *tile_counters = nullptr; /* work-queue heads */
-    size_t tile_counter_cap = 0;
+    void *cont_buf = nullptr;          /* rays handed between render launches */
+    size_t cont_buf_bytes = 0;
+    void *cont_counts = nullptr;       /* per-view counts of handed-over rays */
+    size_t cont_counts_bytes = 0;
 
     LyapTuning tuned{};                /* autotuner's choice, zero = unset */
+    void *debug_wave_log = nullptr;    /* per-wave render record, when set */
     int num_cus = 0;
 };
 

@@ -76,7 +76,20 @@ struct EvalDesc {
     /* occurrences of letters A..D in the whole window and in its first part */
     float count[4];
     float count_half[4];
+    /* Blocked layout for the period-agnostic kernels: a block is B steps, a
+     * whole number of periods that fits the kernel's coefficient window W;
+     * position j of a block uses letter wletters[j] = letters[j mod L].
+     * settle = set_blocks * B + set_rem; the accumulation finishes that block
+     * (positions [head_lo, head_hi)), runs acc_blocks blocks, then positions
+     * [0, acc_rem). */
+    int32_t wletters[LYAP_MAX_SEQ];
+    uint32_t W, B;
+    uint32_t set_blocks, set_rem;
+    uint32_t head_lo, head_hi;
+    uint32_t acc_blocks, acc_rem;
 };
+
+inline void rebuild_blocks(EvalDesc *e, uint32_t W);
 
 /* Host: derive the loop bounds. Returns false for an unusable sequence. */
 inline bool make_eval_desc(EvalDesc *e, const LyapParams &prm,
@@ -123,7 +136,30 @@ inline bool make_eval_desc(EvalDesc *e, const LyapParams &prm,
         e->count_half[seq[j]] += (float)e->half_periods;
     }
     for (uint32_t j = 0; j < e->tail; ++j) e->count[seq[j]] += 1.f;
+    rebuild_blocks(e, L <= 16 ? 16u : 32u);
     return true;
+}
+
+/* Host: blocked layout for a coefficient window of W (16 or 32) positions. */
+inline void rebuild_blocks(EvalDesc *e, uint32_t W)
+{
+    const uint32_t L = (uint32_t)e->L;
+    e->W = W;
+    e->B = L * (W / L);
+    for (uint32_t j = 0; j < (uint32_t)LYAP_MAX_SEQ; ++j) e->wletters[j] = e->letters[j % L];
+    e->set_blocks = e->settle / e->B;
+    e->set_rem = e->settle % e->B;
+    uint32_t left = e->accum;
+    e->head_lo = e->head_hi = 0;
+    if (e->set_rem != 0) {
+        const uint32_t h = e->B - e->set_rem;
+        const uint32_t take = left < h ? left : h;
+        e->head_lo = e->set_rem;
+        e->head_hi = e->set_rem + take;
+        left -= take;
+    }
+    e->acc_blocks = left / e->B;
+    e->acc_rem = left % e->B;
 }
 
 /* Lane's value of sequence letter `s` (0..3 = x, y, z, d); s is uniform. */
@@ -396,6 +432,175 @@ __device__ __forceinline__ double lambda_fp64(float xf, float yf, float zf,
             renorm64(st);
         }
     }
+    const double lam = ((double)st.E * kLn2 + log(st.M)) / (double)e.accum;
+    return __builtin_isfinite(lam) ? lam : __builtin_nan("");
+}
+
+/* ======================================================================== */
+/* Period-agnostic forms: one compiled loop for every period L <= W.         */
+/* ======================================================================== */
+
+/* Steps of a block at positions [lo, hi) (FULL: [0, B), where every block
+ * has at least W/2 + 1 steps, so only the upper half is guarded). */
+template <int W, bool FULL>
+__device__ __forceinline__ bool in_block(int j, uint32_t lo, uint32_t hi)
+{
+    if (FULL) return j <= W / 2 || (uint32_t)j < hi;
+    return (uint32_t)j >= lo && (uint32_t)j < hi;
+}
+
+/* FP32 map in u = 1 - 2x: u' = 1 + b (u^2 - 1), b = r/2. */
+__device__ __forceinline__ float umap(float u, float b)
+{
+    return __builtin_fmaf(b, __builtin_fmaf(u, u, -1.0f), 1.0f);
+}
+
+template <int W>
+__device__ __forceinline__ void load_half_r(float (&b)[W], float x, float y,
+                                            float z, const EvalDesc &e)
+{
+    const float hx = 0.5f * x, hy = 0.5f * y, hz = 0.5f * z, hd = 0.5f * e.d;
+#pragma unroll
+    for (int j = 0; j < W; ++j) b[j] = pick(e.wletters[j], hx, hy, hz, hd);
+}
+
+template <int W, bool FULL>
+__device__ __forceinline__ void settle_block(float &u, const float (&b)[W],
+                                             uint32_t lo, uint32_t hi)
+{
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+        if (in_block<W, FULL>(j, lo, hi)) u = umap(u, b[j]);
+}
+
+template <int W, bool FULL>
+__device__ __forceinline__ void product_block(float &u, ProductState &st,
+                                              const float (&b)[W], uint32_t lo,
+                                              uint32_t hi)
+{
+    const float u0 = u;
+    const ProductState s0 = st;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        if (in_block<W, FULL>(j, lo, hi)) {
+            u = umap(u, b[j]);
+            st.M *= __builtin_fabsf(u);
+        }
+    }
+    if (st.M < kUnderflowGuard) {
+        /* |u| <= 1 keeps M monotone within the block, so ending above the
+         * guard means no step went below it; otherwise redo per step */
+        float w = u0;
+        st = s0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            if (in_block<W, FULL>(j, lo, hi)) {
+                w = umap(w, b[j]);
+                st.M *= __builtin_fabsf(w);
+                renorm(st);
+            }
+        }
+    }
+    renorm(st);
+}
+
+/* FP32 product form, any period L <= W. */
+template <int W>
+__device__ __forceinline__ float lambda_product_w(float x, float y, float z,
+                                                  const EvalDesc &e)
+{
+    float b[W];
+    load_half_r<W>(b, x, y, z, e);
+    float u = 0.0f;
+    for (uint32_t k = 0; k < e.set_blocks; ++k) settle_block<W, true>(u, b, 0, e.B);
+    settle_block<W, false>(u, b, 0, e.set_rem);
+    if (__builtin_fabsf(u) < 2e-8f) return 0.0f / (float)e.accum;   /* kernel.cu:138 */
+    ProductState st{1.0f, 0};
+    product_block<W, false>(u, st, b, e.head_lo, e.head_hi);
+    for (uint32_t k = 0; k < e.acc_blocks; ++k) product_block<W, true>(u, st, b, 0, e.B);
+    product_block<W, false>(u, st, b, 0, e.acc_rem);
+    float lg[4];
+    lg[0] = __builtin_amdgcn_logf(__builtin_fabsf(x));
+    lg[1] = __builtin_amdgcn_logf(__builtin_fabsf(y));
+    lg[2] = __builtin_amdgcn_logf(__builtin_fabsf(z));
+    lg[3] = __builtin_amdgcn_logf(__builtin_fabsf(e.d));
+    return finish_product(st, lg, e.count, e.accum);
+}
+
+/* FP32 x-form with one log2 per accumulated step (the reference method). */
+template <int W, bool FULL>
+__device__ __forceinline__ void plain_block(float &v, float &s, const float (&r)[W],
+                                            uint32_t lo, uint32_t hi, bool acc)
+{
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        if (in_block<W, FULL>(j, lo, hi)) {
+            v = r[j] * __builtin_fmaf(-v, v, v);
+            if (acc) s += __builtin_amdgcn_logf(__builtin_fabsf(__builtin_fmaf(-2.0f * r[j], v, r[j])));
+        }
+    }
+}
+
+template <int W>
+__device__ __forceinline__ float lambda_plain_w(float x, float y, float z,
+                                                const EvalDesc &e)
+{
+    float r[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) r[j] = pick(e.wletters[j], x, y, z, e.d);
+    float v = 0.5f, s = 0.0f;
+    for (uint32_t k = 0; k < e.set_blocks; ++k) plain_block<W, true>(v, s, r, 0, e.B, false);
+    plain_block<W, false>(v, s, r, 0, e.set_rem, false);
+    if (v == 0.5f) return 0.0f / (float)e.accum;
+    plain_block<W, false>(v, s, r, e.head_lo, e.head_hi, true);
+    for (uint32_t k = 0; k < e.acc_blocks; ++k) plain_block<W, true>(v, s, r, 0, e.B, true);
+    plain_block<W, false>(v, s, r, 0, e.acc_rem, true);
+    const float lam = s * kLn2f / (float)e.accum;
+    return __builtin_isfinite(lam) ? lam : __builtin_nanf("");
+}
+
+/* FP64 with the reference's map ((r*v)*(1-v), separately rounded, so the
+ * orbit is the IEEE double one); the derivative r(1 - 2v) only feeds the
+ * log sum and is formed with one fma. */
+template <int W, bool FULL, bool ACC>
+__device__ __forceinline__ void fp64_block(double &v, ProductState64 &st,
+                                           const double (&r)[W],
+                                           const double (&m2r)[W], uint32_t lo,
+                                           uint32_t hi)
+{
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        if (in_block<W, FULL>(j, lo, hi)) {
+            v = (r[j] * v) * (1.0 - v);
+            if (ACC) {
+                st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
+                if (W > 16 && j == 15) renorm64(st);
+            }
+        }
+    }
+    if (ACC) renorm64(st);
+}
+
+template <int W>
+__device__ __forceinline__ double lambda_fp64_w(float xf, float yf, float zf,
+                                                const EvalDesc &e)
+{
+    double r[W], m2r[W];
+    const double x = xf, y = yf, z = zf, d = e.d;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        r[j] = pickd(e.wletters[j], x, y, z, d);
+        m2r[j] = -2.0 * r[j];
+    }
+    double v = 0.5;
+    ProductState64 st{1.0, 0};
+    for (uint32_t k = 0; k < e.set_blocks; ++k) fp64_block<W, true, false>(v, st, r, m2r, 0, e.B);
+    fp64_block<W, false, false>(v, st, r, m2r, 0, e.set_rem);
+    const double off = v - 0.5;
+    if (!(off <= -1e-8 || off >= 1e-8)) return 0.0 / (double)e.accum;
+    fp64_block<W, false, true>(v, st, r, m2r, e.head_lo, e.head_hi);
+    for (uint32_t k = 0; k < e.acc_blocks; ++k) fp64_block<W, true, true>(v, st, r, m2r, 0, e.B);
+    fp64_block<W, false, true>(v, st, r, m2r, 0, e.acc_rem);
     const double lam = ((double)st.E * kLn2 + log(st.M)) / (double)e.accum;
     return __builtin_isfinite(lam) ? lam : __builtin_nan("");
 }

@@ -50,6 +50,19 @@ struct ViewDesc {
     float normal_h;
     float margin;              /* FP64 re-evaluation margin during bisection */
     uint32_t max_samples;
+    uint32_t cont_base;        /* this view's region of the continuation buffers */
+    uint32_t cont_cap;
+};
+
+/* March state of a ray handed to the next launch (80 bytes). */
+struct ContEntry {
+    uint32_t pix, evals, index, flags;
+    float Vx, Vy, Vz, Px, Py, Pz;
+    float t, t1, dt, Fdt, Ndt, last_step;
+    float a, c, l;
+    float tb, dtb;
+    uint32_t m;
+    uint32_t pad[2];
 };
 
 struct RenderArgs {
@@ -57,9 +70,22 @@ struct RenderArgs {
     uint32_t n_views;
     uint32_t total_tiles;
     uint32_t tile_rays;
-    uint32_t pad;
+    uint32_t lanes_per_ray;    /* lanes marching one ray while rays remain */
     uint32_t *tile_counter;
     unsigned long long *eval_counter;
+    /* optional per-wave record {rounds, rays, start clock, end clock} */
+    unsigned long long *wave_log;
+    /* Launch level: 0 takes rays from pixel tiles; a later level takes the
+     * rays the previous launch handed over. A ray that has taken `budget`
+     * march samples in this launch is handed to the next (0: never). */
+    uint32_t level;
+    uint32_t budget;
+    uint32_t resident_lanes;
+    uint32_t pad2;
+    const ContEntry *cont_in;
+    const uint32_t *in_counts;         /* per view */
+    ContEntry *cont_out;
+    uint32_t *out_counts;              /* per view */
 };
 
 namespace {
@@ -182,52 +208,31 @@ __device__ bool clip_box(F3 C, F3 V, float lo, float hi, float &t0_out, float &t
     return true;
 }
 
-/* ---- lambda dispatch over the sequence period --------------------------- */
+/* ---- lambda in the kernel's window ------------------------------------- */
 
-#define LYAP_L_SWITCH(EXPR)                                                    \
-    switch (e.L) {                                                             \
-    case 1: { constexpr int L = 1; EXPR; }   case 2: { constexpr int L = 2; EXPR; }   \
-    case 3: { constexpr int L = 3; EXPR; }   case 4: { constexpr int L = 4; EXPR; }   \
-    case 5: { constexpr int L = 5; EXPR; }   case 6: { constexpr int L = 6; EXPR; }   \
-    case 7: { constexpr int L = 7; EXPR; }   case 8: { constexpr int L = 8; EXPR; }   \
-    case 9: { constexpr int L = 9; EXPR; }   case 10: { constexpr int L = 10; EXPR; } \
-    case 11: { constexpr int L = 11; EXPR; } case 12: { constexpr int L = 12; EXPR; } \
-    case 13: { constexpr int L = 13; EXPR; } case 14: { constexpr int L = 14; EXPR; } \
-    case 15: { constexpr int L = 15; EXPR; } case 16: { constexpr int L = 16; EXPR; } \
-    case 17: { constexpr int L = 17; EXPR; } case 18: { constexpr int L = 18; EXPR; } \
-    case 19: { constexpr int L = 19; EXPR; } case 20: { constexpr int L = 20; EXPR; } \
-    case 21: { constexpr int L = 21; EXPR; } case 22: { constexpr int L = 22; EXPR; } \
-    case 23: { constexpr int L = 23; EXPR; } case 24: { constexpr int L = 24; EXPR; } \
-    case 25: { constexpr int L = 25; EXPR; } case 26: { constexpr int L = 26; EXPR; } \
-    case 27: { constexpr int L = 27; EXPR; } case 28: { constexpr int L = 28; EXPR; } \
-    case 29: { constexpr int L = 29; EXPR; } case 30: { constexpr int L = 30; EXPR; } \
-    case 31: { constexpr int L = 31; EXPR; } default: { constexpr int L = 32; EXPR; } \
-    }
-
-__device__ __noinline__ float eval_fp64(const EvalDesc &e, float x, float y, float z)
+template <int W>
+__device__ __noinline__ float eval_fp64_call(const EvalDesc &e, float x, float y, float z)
 {
-    LYAP_L_SWITCH(return (float)lambda_fp64<L>(x, y, z, e))
+    return (float)lambda_fp64_w<W>(x, y, z, e);
 }
 
-template <int PREC>
+template <int W, int PREC>
 __device__ __forceinline__ float eval_lambda(const EvalDesc &e, float x, float y, float z)
 {
-    if (PREC == LYAP_PREC_FP64) return eval_fp64(e, x, y, z);
-    if (PREC == LYAP_PREC_PLAIN) {
-        LYAP_L_SWITCH(return lambda_plain<L>(x, y, z, e))
-    } else {
-        LYAP_L_SWITCH(return (lambda_product<L, false>(x, y, z, e)).lam)
-    }
+    if (PREC == LYAP_PREC_FP64) return (float)lambda_fp64_w<W>(x, y, z, e);
+    if (PREC == LYAP_PREC_PLAIN) return lambda_plain_w<W>(x, y, z, e);
+    return lambda_product_w<W>(x, y, z, e);
 }
 
-/* Per-lane ray state. */
+/* Per-lane ray state. Every lane of a group holds the same copy; the group's
+ * lanes differ only in which point they evaluate. */
 struct Ray {
     int phase;
     uint32_t pix;          /* pixel index x + y*w */
-    uint32_t evals;
-    uint32_t index;        /* march sample index */
+    uint32_t evals;        /* lambda evaluations the method needed */
+    uint32_t index;        /* march sample index of (t, P) */
     uint32_t k;            /* refine or normal step counter */
-    bool near_, sign;
+    uint32_t flags;        /* bit 0: near step, bit 1: walk sign */
     F3 V, P;
     float t, t1, dt, Fdt, Ndt, last_step;
     float a, c, l;
@@ -236,7 +241,41 @@ struct Ray {
     F3 QdV;
     float h;
     float gx, gy, gz;      /* central differences */
+    uint32_t launch_samples;   /* march samples taken in this launch */
+    /* direct sampling (bisection method): t = tb + m * dtb, rebased when the
+     * step length changes */
+    float tb, dtb;
+    uint32_t m;
 };
+
+__device__ __forceinline__ float shf(float v, int src) { return __shfl(v, src); }
+__device__ __forceinline__ uint32_t shu(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src); }
+
+/* Copy of lane `src`'s ray state. */
+__device__ __forceinline__ Ray shuffle_ray(const Ray &r, int src)
+{
+    Ray o;
+    o.phase = __shfl(r.phase, src);
+    o.pix = shu(r.pix, src);
+    o.evals = shu(r.evals, src);
+    o.index = shu(r.index, src);
+    o.k = shu(r.k, src);
+    o.flags = shu(r.flags, src);
+    o.V = {shf(r.V.x, src), shf(r.V.y, src), shf(r.V.z, src)};
+    o.P = {shf(r.P.x, src), shf(r.P.y, src), shf(r.P.z, src)};
+    o.t = shf(r.t, src); o.t1 = shf(r.t1, src); o.dt = shf(r.dt, src);
+    o.Fdt = shf(r.Fdt, src); o.Ndt = shf(r.Ndt, src); o.last_step = shf(r.last_step, src);
+    o.a = shf(r.a, src); o.c = shf(r.c, src); o.l = shf(r.l, src);
+    o.lo = shf(r.lo, src); o.hi = shf(r.hi, src); o.l_hi = shf(r.l_hi, src);
+    o.Qdt = shf(r.Qdt, src); o.Qt0 = shf(r.Qt0, src); o.Qt1 = shf(r.Qt1, src);
+    o.minQdt = shf(r.minQdt, src);
+    o.QdV = {shf(r.QdV.x, src), shf(r.QdV.y, src), shf(r.QdV.z, src)};
+    o.h = shf(r.h, src);
+    o.gx = shf(r.gx, src); o.gy = shf(r.gy, src); o.gz = shf(r.gz, src);
+    o.launch_samples = shu(r.launch_samples, src);
+    o.tb = shf(r.tb, src); o.dtb = shf(r.dtb, src); o.m = shu(r.m, src);
+    return o;
+}
 
 __device__ __forceinline__ void write_miss(const ViewDesc &v, const Ray &r)
 {
@@ -279,8 +318,8 @@ __device__ void write_hit(const ViewDesc &v, const Ray &r, F3 N)
     v.rgba[r.pix] = o;
 }
 
-/* Starts ray `id` of view v; returns false (after writing a miss or nothing
- * for a padding ray) when no lambda evaluation is needed. */
+/* Starts ray `id` of view v; returns false (after writing a miss, or nothing
+ * for a padding ray past the image edge) when no evaluation is needed. */
 __device__ bool start_ray(const ViewDesc &v, uint32_t id, Ray &r)
 {
     const uint32_t blk = id >> 6, in = id & 63u;
@@ -304,37 +343,78 @@ __device__ bool start_ray(const ViewDesc &v, uint32_t id, Ray &r)
     r.P = add(C, mul(V, t0));
     r.a = 0.f;
     r.c = 0.f;
+    r.l = 0.f;
     if (v.prm.stepMethod == 1) r.Fdt = (t1 - t0) / v.prm.depth;
     else r.Fdt = sqrtf(mag2(V)) / v.prm.depth;
     r.dt = r.Fdt;
     r.Ndt = r.dt / v.prm.nearMultiplier;
-    r.near_ = false;
+    r.flags = 0;
     r.last_step = r.dt;
     r.index = 0;
+    r.k = 0;
+    r.launch_samples = 0;
+    r.tb = t0;
+    r.dtb = r.dt;
+    r.m = 0;
     r.phase = PH_MARCH;
     return true;
 }
 
-/* Point the lane evaluates this trip (advances the walk first). */
-__device__ __forceinline__ F3 sample_point(const ViewDesc &v, Ray &r)
+/* Resumes a ray handed over by the previous launch. */
+__device__ __forceinline__ void resume_ray(const ContEntry &c, Ray &r)
 {
-    if (r.phase == PH_MARCH) return r.P;
-    if (r.phase == PH_REFINE) {
-        if (v.refine == LYAP_REFINE_BISECT) {
-            const float mid = r.lo + (r.hi - r.lo) * 0.5f;
-            return add(f3(v.cam.C), mul(r.V, mid));
+    r.pix = c.pix;
+    r.evals = c.evals;
+    r.index = c.index;
+    r.flags = c.flags;
+    r.V = {c.Vx, c.Vy, c.Vz};
+    r.P = {c.Px, c.Py, c.Pz};
+    r.t = c.t; r.t1 = c.t1; r.dt = c.dt; r.Fdt = c.Fdt; r.Ndt = c.Ndt;
+    r.last_step = c.last_step;
+    r.a = c.a; r.c = c.c; r.l = c.l;
+    r.tb = c.tb; r.dtb = c.dtb; r.m = c.m;
+    r.k = 0;
+    r.launch_samples = 0;
+    r.phase = PH_MARCH;
+}
+
+__device__ __forceinline__ void suspend_ray(const Ray &r, ContEntry &c)
+{
+    c.pix = r.pix; c.evals = r.evals; c.index = r.index; c.flags = r.flags;
+    c.Vx = r.V.x; c.Vy = r.V.y; c.Vz = r.V.z;
+    c.Px = r.P.x; c.Py = r.P.y; c.Pz = r.P.z;
+    c.t = r.t; c.t1 = r.t1; c.dt = r.dt; c.Fdt = r.Fdt; c.Ndt = r.Ndt;
+    c.last_step = r.last_step;
+    c.a = r.a; c.c = r.c; c.l = r.l;
+    c.tb = r.tb; c.dtb = r.dtb; c.m = r.m;
+    c.pad[0] = c.pad[1] = 0;
+}
+
+/* Position of the next march sample after a step of `step` (the next t and,
+ * in the reference walk method, P += V*step). The bisection method samples
+ * t = tb + m*dtb directly and rebases (tb = t, dtb = step, m = 0) when the
+ * step length changes. Returns false when the step does not change t (a
+ * grazing ray with t1 - t0 near 0, or depth = inf), which would never leave
+ * the box. */
+__device__ __forceinline__ bool march_step(const ViewDesc &v, Ray &r, float step)
+{
+    if (v.refine == LYAP_REFINE_BISECT) {
+        if (step != r.dtb) {
+            r.tb = r.t;
+            r.dtb = step;
+            r.m = 0;
         }
-        r.t += r.Qdt;
-        r.P = add(r.P, r.QdV);
-        return r.P;
+        const float tn = r.tb + (float)(r.m + 1u) * r.dtb;
+        if (!(tn > r.t)) return false;
+        r.t = tn;
+        ++r.m;
+        r.P = add(f3(v.cam.C), mul(r.V, r.t));
+        return true;
     }
-    /* PH_NORMAL: x-, x+, y-, y+, z-, z+ (P + (-h) equals P - h exactly) */
-    F3 s = r.P;
-    const float off = (r.k & 1u) ? r.h : -r.h;
-    if (r.k < 2) s.x = r.P.x + off;
-    else if (r.k < 4) s.y = r.P.y + off;
-    else s.z = r.P.z + off;
-    return s;
+    if (!(r.t + step > r.t)) return false;
+    r.t += step;
+    r.P = add(r.P, mul(r.V, step));
+    return true;
 }
 
 __device__ __forceinline__ void enter_normal(const ViewDesc &v, Ray &r)
@@ -361,7 +441,7 @@ __device__ __forceinline__ void enter_refine(const ViewDesc &v, Ray &r)
         r.l_hi = r.l;
         if (v.bisect_steps <= 0) enter_normal(v, r);
     } else {
-        r.sign = false;
+        r.flags &= ~2u;
         r.Qdt = r.dt * -0.5f;
         r.QdV = mul(r.V, r.Qdt);
         r.Qt1 = r.t;
@@ -371,76 +451,284 @@ __device__ __forceinline__ void enter_refine(const ViewDesc &v, Ray &r)
     }
 }
 
-/* Consumes the lambda of this trip's sample. Returns true when the ray is
- * finished (outputs written). */
-__device__ bool advance(const ViewDesc &v, Ray &r, float l)
+/* Bisection levels a group of K lanes resolves per trip: the 2^m - 1 nodes
+ * of the next m levels of the bisection tree. */
+__device__ __forceinline__ uint32_t bisect_levels(uint32_t K, uint32_t left)
 {
-    ++r.evals;
-    const LyapParams &prm = v.prm;
+    uint32_t m = 31u - (uint32_t)__builtin_clz(K + 1u);   /* floor(log2(K+1)) */
+    return m < left ? m : left;
+}
+
+/* Point lane `rank` of the group evaluates this trip; false when it has
+ * nothing to evaluate. Positions repeat the sequential method's float
+ * operations exactly. */
+__device__ __forceinline__ bool sample_point(const ViewDesc &v, const Ray &r,
+                                             uint32_t rank, uint32_t K, F3 &s,
+                                             float &ts)
+{
     if (r.phase == PH_MARCH) {
-        r.l = l;
-        if (r.index > 0) {
-            if (l > prm.chaosThreshold) r.c += l;
-            else if (l > prm.opaqueThreshold) r.a += l;
-            if (l <= prm.nearThreshold && !r.near_) { r.near_ = true; r.dt = r.Ndt; }
-            else if (l > prm.nearThreshold && r.near_) { r.near_ = false; r.dt = r.Fdt; }
+        ts = r.t;
+        if (rank == 0) { s = r.P; return true; }
+        if (v.prm.jitter != 0.0f) return false;       /* next step needs this lambda */
+        if (r.index + rank >= v.max_samples) return false;
+        if (v.refine == LYAP_REFINE_BISECT) {
+            /* rank steps ahead at the current step length */
+            const float tp = r.tb + (float)(r.m + rank - 1u) * r.dt;
+            const float t = r.tb + (float)(r.m + rank) * r.dt;
+            if (r.dt != r.dtb || !(t > tp) || t > r.t1) return false;
+            ts = t;
+            s = add(f3(v.cam.C), mul(r.V, t));
+            return true;
         }
-        if (l > prm.opaqueThreshold) {
-            float step = r.dt;
-            if (prm.jitter != 0.0f) {
-                float jit = l - truncf(l);
-                if (jit < 0) jit = (float)(1.0 - (double)(jit * prm.jitter));
-                else jit = (float)(1.0 + (double)(jit * prm.jitter));
-                if (__builtin_isfinite(jit)) step = r.dt * jit;
-            }
-            r.t += step;
-            if (v.refine == LYAP_REFINE_BISECT) r.P = add(f3(v.cam.C), mul(r.V, r.t));
-            else r.P = add(r.P, mul(r.V, step));
-            r.last_step = step;
-            ++r.index;
-            if (r.t > r.t1 || r.index >= v.max_samples) {
-                write_miss(v, r);
-                return true;
-            }
-            return false;
+        /* reference walk method: rank increments of dt */
+        float t = r.t;
+        F3 P = r.P;
+        for (uint32_t j = 0; j < rank; ++j) {
+            if (!(t + r.dt > t)) return false;
+            t += r.dt;
+            P = add(P, mul(r.V, r.dt));
         }
-        enter_refine(v, r);
-        return false;
+        if (t > r.t1) return false;
+        ts = t;
+        s = P;
+        return true;
     }
     if (r.phase == PH_REFINE) {
         if (v.refine == LYAP_REFINE_BISECT) {
-            const float mid = r.lo + (r.hi - r.lo) * 0.5f;
-            if (!(l > prm.opaqueThreshold)) { r.hi = mid; r.l_hi = l; }
-            else r.lo = mid;
-            if ((int)++r.k >= v.bisect_steps) {
-                r.t = r.hi;
-                r.P = add(f3(v.cam.C), mul(r.V, r.t));
-                r.l = r.l_hi;
-                enter_normal(v, r);
+            const uint32_t m = bisect_levels(K, (uint32_t)v.bisect_steps - r.k);
+            if (rank >= (1u << m) - 1u) return false;
+            /* node rank of the next m levels: depth d, path bits below the top */
+            const uint32_t n1 = rank + 1u;
+            const uint32_t d = 31u - (uint32_t)__builtin_clz(n1);
+            float lo = r.lo, hi = r.hi;
+            for (int i = (int)d - 1; i >= 0; --i) {
+                const float mid = lo + (hi - lo) * 0.5f;
+                if ((n1 >> i) & 1u) lo = mid;   /* transparent branch */
+                else hi = mid;                  /* opaque branch */
             }
-            return false;
+            const float mid = lo + (hi - lo) * 0.5f;
+            s = add(f3(v.cam.C), mul(r.V, mid));
+            return true;
         }
-        r.l = l;
-        ++r.k;
-        if (l == prm.opaqueThreshold) { enter_normal(v, r); return false; }
-        const bool osign = r.sign;
-        r.sign = (l < prm.opaqueThreshold) ? false : true;
-        if (r.sign != osign) {
-            r.Qdt *= -0.5f;
-            r.QdV = mul(r.QdV, -0.5f);
-        }
-        if (!walk_continues(r)) enter_normal(v, r);
-        return false;
+        if (rank != 0) return false;
+        s = add(r.P, r.QdV);                            /* walk: t += Qdt; P += QdV */
+        return true;
     }
-    /* PH_NORMAL: g = l(+h) - l(-h) per axis, formed as (-l(-h)) + l(+h) */
-    const float sl = (r.k & 1u) ? l : -l;
-    if (r.k < 2) r.gx = (r.k == 0) ? sl : r.gx + sl;
-    else if (r.k < 4) r.gy = (r.k == 2) ? sl : r.gy + sl;
-    else r.gz = (r.k == 4) ? sl : r.gz + sl;
-    if (++r.k < 6) return false;
-    const F3 N = normalized({r.gx, r.gy, r.gz});
-    write_hit(v, r, N);
+    /* PH_NORMAL: samples k.. in the order x-, x+, y-, y+, z-, z+ */
+    const uint32_t n = r.k + rank;
+    if (n >= 6) return false;
+    s = r.P;
+    const float off = (n & 1u) ? r.h : -r.h;
+    if (n < 2) s.x = r.P.x + off;
+    else if (n < 4) s.y = r.P.y + off;
+    else s.z = r.P.z + off;
     return true;
+}
+
+/* Consumes one march sample (the lambda at (t, P)). Returns 0 to go on,
+ * 1 when the ray left the box or hit (state updated), 2 when the step length
+ * changed so later speculative samples no longer apply. */
+__device__ __forceinline__ int march_consume(const ViewDesc &v, Ray &r, float l,
+                                             bool leader)
+{
+    const LyapParams &prm = v.prm;
+    ++r.evals;
+    ++r.launch_samples;
+    r.l = l;
+    bool toggled = false;
+    if (r.index > 0) {
+        /* c += l above the chaos threshold, else a += l above the opaque one
+         * (kernel.cu:370-375); adding +0 leaves either sum unchanged */
+        const bool to_c = l > prm.chaosThreshold;
+        const bool to_a = !to_c && l > prm.opaqueThreshold;
+        r.c += to_c ? l : 0.0f;
+        r.a += to_a ? l : 0.0f;
+        const bool near_ = (r.flags & 1u) != 0;
+        if (l <= prm.nearThreshold && !near_) { r.flags |= 1u; r.dt = r.Ndt; toggled = r.Ndt != r.Fdt; }
+        else if (l > prm.nearThreshold && near_) { r.flags &= ~1u; r.dt = r.Fdt; toggled = r.Ndt != r.Fdt; }
+    }
+    if (l > prm.opaqueThreshold) {
+        float step = r.dt;
+        if (prm.jitter != 0.0f) {
+            float jit = l - truncf(l);
+            if (jit < 0) jit = (float)(1.0 - (double)(jit * prm.jitter));
+            else jit = (float)(1.0 + (double)(jit * prm.jitter));
+            if (__builtin_isfinite(jit)) step = r.dt * jit;
+        }
+        if (!march_step(v, r, step)) {
+            if (leader) write_miss(v, r);
+            r.phase = PH_IDLE;
+            return 1;
+        }
+        r.last_step = step;
+        ++r.index;
+        if (r.t > r.t1 || r.index >= v.max_samples) {
+            if (leader) write_miss(v, r);
+            r.phase = PH_IDLE;
+            return 1;
+        }
+        /* under jitter only the first lane evaluated (its step needs lambda) */
+        return (toggled || prm.jitter != 0.0f) ? 2 : 0;
+    }
+    enter_refine(v, r);
+    return 1;
+}
+
+/* Combines the group's lambdas (lane gbase + j holds the j-th point) and
+ * advances the replicated state in the sequential method's order. All lanes
+ * run the same uniform loops so every shuffle sees the whole wave. Returns
+ * true when this lane's ray finished this trip. */
+__device__ bool combine(const ViewDesc &v, Ray &r, float lam, bool valid,
+                        float tsamp, F3 psamp, uint32_t gbase, uint32_t K,
+                        bool busy, bool leader)
+{
+    const int ph = busy ? r.phase : PH_IDLE;
+    const float thr = v.prm.opaqueThreshold;
+    const bool bisect = v.refine == LYAP_REFINE_BISECT;
+    bool finished = false;
+
+    /* march: the group's samples j = 0..K-1 were taken at the positions the
+     * sequential method would reach if none of them stopped it. Find the
+     * first that does (opaque, past the exit, or a change of step length),
+     * account the transparent ones before it in order, then consume it. */
+    if (__ballot(ph == PH_MARCH) != 0) {
+        const uint32_t rank = __lane_id() - gbase;
+        const bool marching = ph == PH_MARCH;
+        bool ev = true;
+        if (marching && valid) {
+            const LyapParams &prm = v.prm;
+            const bool near_ = (r.flags & 1u) != 0;
+            const bool toggles = (r.index + rank) > 0 && r.Ndt != r.Fdt &&
+                                 ((lam <= prm.nearThreshold) != near_);
+            ev = !(lam > prm.opaqueThreshold) || toggles || prm.jitter != 0.0f;
+        }
+        const uint64_t evm = __ballot(ev);
+        const uint64_t gmask = (K >= 64u) ? ~0ull : (((1ull << K) - 1ull) << gbase);
+        const uint64_t mine = (evm & gmask) >> gbase;
+        const uint32_t e = mine ? (uint32_t)__ffsll((unsigned long long)mine) - 1u : K;
+        /* transparent samples 0..e-1, summed in order (kernel.cu:370-375) */
+        for (uint32_t j = 0; __ballot(marching && j < e) != 0; ++j) {
+            const float lj = shf(lam, (int)(gbase + j));
+            if (marching && j < e) {
+                if (r.index > 0) {
+                    const bool to_c = lj > v.prm.chaosThreshold;
+                    const bool to_a = !to_c && lj > v.prm.opaqueThreshold;
+                    r.c += to_c ? lj : 0.0f;
+                    r.a += to_a ? lj : 0.0f;
+                }
+                ++r.evals;
+                ++r.launch_samples;
+                ++r.index;
+                r.l = lj;
+            }
+        }
+        /* position and lambda of sample e */
+        const uint32_t src = gbase + (e < K ? e : K - 1u);
+        const float te = shf(tsamp, (int)src);
+        const F3 Pe = {shf(psamp.x, (int)src), shf(psamp.y, (int)src), shf(psamp.z, (int)src)};
+        const float le = shf(lam, (int)src);
+        const uint32_t ve = shu(valid ? 1u : 0u, (int)src);
+        if (marching) {
+            if (e > 0) {
+                r.last_step = r.dt;
+                if (v.refine == LYAP_REFINE_BISECT) r.m += (e < K ? e : K - 1u);
+            }
+            if (e < K) {
+                r.t = te;
+                r.P = Pe;
+                if (!ve) {
+                    /* the step to sample e left the box or could not advance */
+                    if (leader) write_miss(v, r);
+                    r.phase = PH_IDLE;
+                    finished = true;
+                } else {
+                    march_consume(v, r, le, leader);
+                    if (r.phase == PH_IDLE) finished = true;
+                }
+            } else {
+                /* all K transparent: step from sample K-1 to sample K */
+                r.t = te;
+                r.P = Pe;
+                if (r.index >= v.max_samples || !march_step(v, r, r.dt) || r.t > r.t1) {
+                    if (leader) write_miss(v, r);
+                    r.phase = PH_IDLE;
+                    finished = true;
+                }
+            }
+        }
+    }
+
+    /* bisection: follow the actual path through this trip's tree levels */
+    if (__ballot(ph == PH_REFINE && bisect) != 0) {
+        const uint32_t m = (ph == PH_REFINE && bisect)
+                               ? bisect_levels(K, (uint32_t)v.bisect_steps - r.k) : 0u;
+        uint32_t node = 0;
+        for (uint32_t i = 0; __ballot(i < m) != 0; ++i) {
+            const float ln = shf(lam, (int)(gbase + node));
+            if (i < m) {
+                const float mid = r.lo + (r.hi - r.lo) * 0.5f;
+                ++r.evals;
+                ++r.k;
+                if (!(ln > thr)) { r.hi = mid; r.l_hi = ln; node = 2u * node + 1u; }
+                else { r.lo = mid; node = 2u * node + 2u; }
+            }
+        }
+        if (ph == PH_REFINE && bisect && (int)r.k >= v.bisect_steps) {
+            r.t = r.hi;
+            r.P = add(f3(v.cam.C), mul(r.V, r.t));
+            r.l = r.l_hi;
+            enter_normal(v, r);
+        }
+    }
+
+    /* reversing walk (kernel.cu:420-441): one sample, by the first lane */
+    if (__ballot(ph == PH_REFINE && !bisect) != 0) {
+        const float l0 = shf(lam, (int)gbase);
+        if (ph == PH_REFINE && !bisect) {
+            r.t += r.Qdt;
+            r.P = add(r.P, r.QdV);
+            ++r.evals;
+            ++r.k;
+            r.l = l0;
+            if (l0 == thr) {
+                enter_normal(v, r);
+            } else {
+                const bool osign = (r.flags & 2u) != 0;
+                const bool sign = (l0 < thr) ? false : true;
+                if (sign) r.flags |= 2u;
+                else r.flags &= ~2u;
+                if (sign != osign) {
+                    r.Qdt *= -0.5f;
+                    r.QdV = mul(r.QdV, -0.5f);
+                }
+                if (!walk_continues(r)) enter_normal(v, r);
+            }
+        }
+    }
+
+    /* normal: the six samples in order (kernel.cu:456-473) */
+    if (__ballot(ph == PH_NORMAL) != 0) {
+        const uint32_t n6 = K < 6u ? K : 6u;
+        for (uint32_t j = 0; j < n6; ++j) {
+            const float lj = shf(lam, (int)(gbase + j));
+            if (ph == PH_NORMAL && r.k < 6u) {
+                const uint32_t n = r.k;
+                const float sl = (n & 1u) ? lj : -lj;
+                if (n < 2) r.gx = (n == 0) ? sl : r.gx + sl;
+                else if (n < 4) r.gy = (n == 2) ? sl : r.gy + sl;
+                else r.gz = (n == 4) ? sl : r.gz + sl;
+                ++r.k;
+                ++r.evals;
+            }
+        }
+        if (ph == PH_NORMAL && r.k >= 6u) {
+            const F3 N = normalized({r.gx, r.gy, r.gz});
+            if (leader) write_hit(v, r, N);
+            r.phase = PH_IDLE;
+            finished = true;
+        }
+    }
+    return finished;
 }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t x)
@@ -448,87 +736,203 @@ __device__ __forceinline__ uint32_t uniform(uint32_t x)
     return __builtin_amdgcn_readfirstlane(x);
 }
 
-/* View owning tile t (views are few; scalar scan). */
-__device__ __forceinline__ uint32_t view_of_tile(const RenderArgs &a, uint32_t t)
+/* Rays [first, last) of view `view` for tile t. Level 0 tiles cover the
+ * views' pixel blocks; later levels cover each view's handed-over rays,
+ * `tile` entries per tile. Views are few: scalar scans. Returns false when t
+ * is past the last tile. */
+__device__ __forceinline__ bool tile_lookup(const ViewDesc *__restrict__ views,
+                                            const RenderArgs &a, uint32_t t,
+                                            uint32_t tile, uint32_t &view,
+                                            uint32_t &first, uint32_t &last)
 {
-    uint32_t v = 0;
-    while (v + 1 < a.n_views && a.views[v + 1].tile_begin <= t) ++v;
-    return uniform(v);
+    if (a.level == 0) {
+        if (t >= a.total_tiles) return false;
+        uint32_t v = 0;
+        while (v + 1 < a.n_views && views[uniform(v + 1)].tile_begin <= t) ++v;
+        view = uniform(v);
+        const ViewDesc &d = views[view];
+        first = uniform((t - d.tile_begin) * tile);
+        last = uniform(min(first + tile, d.n_rays));
+        return true;
+    }
+    uint32_t acc = 0;
+    for (uint32_t v = 0; v < a.n_views; ++v) {
+        const uint32_t cnt = uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+        const uint32_t nt = (cnt + tile - 1) / tile;
+        if (t < acc + nt) {
+            view = uniform(v);
+            first = uniform((t - acc) * tile);
+            last = uniform(min(first + tile, cnt));
+            return true;
+        }
+        acc += nt;
+    }
+    return false;
 }
 
-template <int PREC>
-__global__ __launch_bounds__(256) void render_persistent(RenderArgs a)
+template <int W, int PREC, bool MARGIN>
+__global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restrict__ views,
+                                                         RenderArgs a)
 {
+    const uint32_t lane = __lane_id();
     Ray r;
     r.phase = PH_IDLE;
+    r.pix = 0;
+    r.evals = 0;
     uint32_t done_evals = 0;
 
+    uint32_t K0 = a.lanes_per_ray;    /* lanes per ray while rays remain */
+    uint32_t tile = a.tile_rays;
+    if (a.level > 0) {
+        /* size the groups to the number of rays handed over */
+        uint32_t total = 0;
+        for (uint32_t v = 0; v < a.n_views; ++v)
+            total += uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+        if (total == 0) return;
+        K0 = 1;
+        while (K0 < 64u && (uint64_t)a.resident_lanes >= (uint64_t)total * (2u * K0)) K0 *= 2;
+        tile = uniform(64u / K0);
+    }
+    K0 = uniform(K0);
+    uint32_t K = K0;                  /* lanes per ray; wave-uniform */
+    const unsigned long long t_start = a.wave_log ? clock64() : 0ull;
+    uint32_t rounds = 0, rays_started = 0;
     uint32_t cur_view = 0xFFFFFFFFu;  /* view the lanes work on */
     uint32_t pos = 0, end = 0;        /* unstarted rays of the current tile */
     uint32_t pend_tile = 0xFFFFFFFFu; /* claimed tile of another view */
     bool queue_empty = false;
 
     for (;;) {
-        /* refill idle lanes */
+        /* ---- refill idle groups (one ray per group of K lanes) ---------- */
+        bool started = false;
         for (;;) {
-            const bool need = r.phase == PH_IDLE;
+            const uint32_t rank = lane % K;
+            const bool need = rank == 0 && lane + K <= 64u && r.phase == PH_IDLE;
             const uint64_t mask = __ballot(need);
             if (mask == 0) break;
             if (pos >= end) {
                 if (pend_tile != 0xFFFFFFFFu) {
-                    /* switch views only once every lane is idle */
-                    if (__ballot(r.phase != PH_IDLE) != 0) break;
-                    const uint32_t t = pend_tile;
+                    if (__ballot(r.phase != PH_IDLE) != 0) break;   /* drain first */
+                    uint32_t tv, f, l;
+                    tile_lookup(views, a, pend_tile, tile, tv, f, l);
                     pend_tile = 0xFFFFFFFFu;
-                    cur_view = view_of_tile(a, t);
-                    const ViewDesc &v = a.views[cur_view];
-                    pos = (t - v.tile_begin) * a.tile_rays;
-                    end = uniform(min(pos + a.tile_rays, v.n_rays));
+                    cur_view = tv;
+                    pos = f;
+                    end = l;
+                    K = K0;
                     continue;
                 }
                 if (queue_empty) break;
                 uint32_t t = 0;
-                if (__lane_id() == 0) t = atomicAdd(a.tile_counter, 1u);
-                t = uniform(__shfl(t, 0));
-                if (t >= a.total_tiles) { queue_empty = true; break; }
-                const uint32_t tv = view_of_tile(a, t);
+                if (lane == 0) t = atomicAdd(a.tile_counter, 1u);
+                t = uniform((uint32_t)__shfl((int)t, 0));
+                uint32_t tv, f, l;
+                if (!tile_lookup(views, a, t, tile, tv, f, l)) { queue_empty = true; break; }
                 if (tv != cur_view) { pend_tile = t; continue; }
-                const ViewDesc &v = a.views[cur_view];
-                pos = (t - v.tile_begin) * a.tile_rays;
-                end = uniform(min(pos + a.tile_rays, v.n_rays));
+                pos = f;
+                end = l;
                 continue;
             }
             const uint32_t avail = end - pos;
-            const uint32_t rank = (uint32_t)__popcll(mask & __lanemask_lt());
-            const ViewDesc &v = a.views[cur_view];
-            if (need && rank < avail) {
-                if (!start_ray(v, pos + rank, r)) r.phase = PH_IDLE;
+            const uint32_t rk = (uint32_t)__popcll(mask & __lanemask_lt());
+            if (need && rk < avail) {
+                const ViewDesc &v = views[uniform(cur_view)];
+                if (a.level == 0) {
+                    if (start_ray(v, pos + rk, r)) started = true;
+                    else r.phase = PH_IDLE;
+                } else {
+                    resume_ray(a.cont_in[v.cont_base + pos + rk], r);
+                    started = true;
+                }
             }
             const uint32_t taken = (uint32_t)__popcll(mask);
             pos += taken < avail ? taken : avail;
         }
-        if (__ballot(r.phase != PH_IDLE) == 0) {
+        const uint32_t gbase0 = lane - lane % K;
+        rays_started += (uint32_t)__popcll(__ballot(started));
+        if (__ballot(started) != 0) {
+            /* members mirror their group's first lane */
+            Ray c = shuffle_ray(r, (int)gbase0);
+            r = c;
+        }
+        const uint64_t busy_mask = __ballot(r.phase != PH_IDLE);
+        if (busy_mask == 0) {
             if (queue_empty && pend_tile == 0xFFFFFFFFu) break;
             continue;
         }
-        const ViewDesc &v = a.views[cur_view];
-        if (r.phase != PH_IDLE) {
-            const F3 s = sample_point(v, r);
-            float l = eval_lambda<PREC>(v.e, s.x, s.y, s.z);
-            if (PREC != LYAP_PREC_FP64 && v.margin > 0.0f && r.phase == PH_REFINE &&
+        /* ---- no more rays for this view: spread idle lanes over busy rays */
+        const bool starved = pos >= end && (queue_empty || pend_tile != 0xFFFFFFFFu);
+        if (starved) {
+            const uint64_t leaders = __ballot(r.phase != PH_IDLE && lane % K == 0u);
+            const uint32_t nb = (uint32_t)__popcll(leaders);
+            uint32_t K2 = 64u / nb;
+            if (K2 > 64u) K2 = 64u;
+            if (K2 > K) {
+                /* group i of the new layout takes the i-th busy ray */
+                uint32_t src = lane;
+                bool keep = false;
+                uint64_t m = leaders;
+                for (uint32_t i = 0; i < nb; ++i) {
+                    const uint32_t b = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    m &= m - 1;
+                    if (lane / K2 == i) { src = b; keep = true; }
+                }
+                Ray c = shuffle_ray(r, (int)src);
+                r = c;
+                if (!keep) r.phase = PH_IDLE;
+                K = K2;
+            }
+        }
+        ++rounds;
+        /* ---- one lambda evaluation per lane ----------------------------- */
+        const ViewDesc &v = views[uniform(cur_view)];
+        const uint32_t rank = lane % K;
+        const uint32_t gbase = lane - rank;
+        const bool busy = r.phase != PH_IDLE;
+        float lam = NAN;
+        F3 s = {0.f, 0.f, 0.f};
+        float ts = 0.f;
+        const bool valid = busy && sample_point(v, r, rank, K, s, ts);
+        if (valid) {
+            lam = eval_lambda<W, PREC>(v.e, s.x, s.y, s.z);
+            if (MARGIN && PREC != LYAP_PREC_FP64 && r.phase == PH_REFINE &&
                 v.refine == LYAP_REFINE_BISECT &&
-                !(__builtin_fabsf(l - v.prm.opaqueThreshold) >= v.margin))
-                l = eval_fp64(v.e, s.x, s.y, s.z);
-            if (advance(v, r, l)) {
-                done_evals += r.evals;
-                r.phase = PH_IDLE;
+                !(__builtin_fabsf(lam - v.prm.opaqueThreshold) >= v.margin))
+                lam = eval_fp64_call<W>(v.e, s.x, s.y, s.z);
+        }
+        if (combine(v, r, lam, valid, ts, s, gbase, K, busy, rank == 0) && rank == 0)
+            done_evals += r.evals;
+        /* hand rays that used this launch's sample budget to the next launch */
+        if (a.budget != 0) {
+            const bool over = r.phase == PH_MARCH && r.launch_samples >= a.budget;
+            if (__ballot(over) != 0) {
+                uint32_t ok = 0;
+                if (over && rank == 0) {
+                    const uint32_t slot = atomicAdd(&a.out_counts[uniform(cur_view)], 1u);
+                    if (slot < v.cont_cap) {
+                        ContEntry c;
+                        suspend_ray(r, c);
+                        a.cont_out[v.cont_base + slot] = c;
+                        ok = 1;
+                    }
+                }
+                ok = (uint32_t)__shfl((int)ok, (int)gbase);
+                if (over && ok) r.phase = PH_IDLE;
             }
         }
     }
     /* one 64-bit atomic per wave for the evaluation count */
     unsigned long long sum = done_evals;
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
-    if (__lane_id() == 0 && sum) atomicAdd(a.eval_counter, sum);
+    if (lane == 0 && sum) atomicAdd(a.eval_counter, sum);
+    if (a.wave_log && lane == 0) {
+        const uint32_t wid = blockIdx.x * 4u + threadIdx.x / 64u;
+        unsigned long long *o = a.wave_log + 4ull * (wid + 8192ull * a.level);
+        o[0] = rounds;
+        o[1] = rays_started;
+        o[2] = t_start;
+        o[3] = clock64();
+    }
 }
 
 } /* namespace */
@@ -593,33 +997,163 @@ static int upload_views(lyap_ctx *ctx, const ViewDesc *views, unsigned n)
     return LYAP_OK;
 }
 
-int launch_views(lyap_ctx *ctx, ViewDesc *views, unsigned n_views,
-                 uint32_t total_tiles, uint32_t tile_rays, uint32_t waves_per_cu,
-                 int precision)
+/* Lanes per ray while the queue still has rays: the power of two that lets
+ * the launch's rays fill the resident lanes once, at least 4 (a group of 4
+ * bisects two levels per trip and takes the six normal samples in two) and
+ * at most 16. */
+static uint32_t default_lanes_per_ray(uint32_t resident_lanes, uint64_t rays)
 {
-    int rc = upload_views(ctx, views, n_views);
+    uint32_t k = 4;
+    while (k < 16u && (uint64_t)resident_lanes >= rays * (uint64_t)(2 * k)) k *= 2;
+    return k;
+}
+
+/* Launch shape for `rays` rays: persistent grid, lanes per ray and the rays
+ * per work-queue claim. A claim is one generation of a wave's groups, so
+ * claims are many times the number of waves and the queue balances. */
+struct LaunchPlan {
+    uint32_t blocks, lanes_per_ray, tile_rays, resident_lanes;
+};
+
+/* March-sample budgets of the launch levels; the last level has none. */
+static const uint32_t kLevelBudget[4] = {32u, 64u, 160u, 0u};
+
+template <int W, int PREC, bool MARGIN>
+static uint32_t resident_blocks_per_cu()
+{
+    static int cached = -1;
+    if (cached < 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void *>(&render_persistent<W, PREC, MARGIN>), 256, 0) != hipSuccess ||
+            n <= 0)
+            n = 2;
+        cached = n;
+    }
+    return (uint32_t)cached;
+}
+
+static uint32_t kernel_blocks_per_cu(uint32_t W, int precision, bool margin)
+{
+    if (W == 16) {
+        if (precision == LYAP_PREC_PLAIN) return resident_blocks_per_cu<16, LYAP_PREC_PLAIN, false>();
+        if (precision == LYAP_PREC_PRODUCT)
+            return margin ? resident_blocks_per_cu<16, LYAP_PREC_PRODUCT, true>()
+                          : resident_blocks_per_cu<16, LYAP_PREC_PRODUCT, false>();
+        return resident_blocks_per_cu<16, LYAP_PREC_FP64, false>();
+    }
+    if (precision == LYAP_PREC_PLAIN) return resident_blocks_per_cu<32, LYAP_PREC_PLAIN, false>();
+    if (precision == LYAP_PREC_PRODUCT)
+        return margin ? resident_blocks_per_cu<32, LYAP_PREC_PRODUCT, true>()
+                      : resident_blocks_per_cu<32, LYAP_PREC_PRODUCT, false>();
+    return resident_blocks_per_cu<32, LYAP_PREC_FP64, false>();
+}
+
+static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, uint32_t lanes_hint,
+                              uint32_t W, int precision, bool margin)
+{
+    LaunchPlan p;
+    const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
+    uint32_t per_cu = kernel_blocks_per_cu(W, precision, margin);
+    if (ctx->tuned.waves_per_cu) per_cu = (ctx->tuned.waves_per_cu + 3u) / 4u;
+    p.blocks = cus * per_cu;
+    p.resident_lanes = p.blocks * 256u;
+    p.lanes_per_ray = lanes_hint ? lanes_hint
+                      : (ctx->tuned.lanes_per_ray ? ctx->tuned.lanes_per_ray
+                                                  : default_lanes_per_ray(p.resident_lanes, rays));
+    if (p.lanes_per_ray > 64u) p.lanes_per_ray = 64u;
+    if (p.lanes_per_ray == 0) p.lanes_per_ray = 1;
+    p.tile_rays = ctx->tuned.tile_rays ? ctx->tuned.tile_rays : 64u / p.lanes_per_ray;
+    return p;
+}
+
+static int ensure_buffer(lyap_ctx *ctx, void **buf, size_t *cap, size_t bytes)
+{
+    if (bytes <= *cap) return LYAP_OK;
+    LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx_free(ctx, *buf);
+    *buf = nullptr;
+    *cap = 0;
+    int rc = ctx_alloc(ctx, buf, bytes);
+    if (rc) return rc;
+    *cap = bytes;
+    return LYAP_OK;
+}
+
+int launch_views(lyap_ctx *ctx, ViewDesc *views, unsigned n_views,
+                 uint32_t total_tiles, const LaunchPlan &plan, uint32_t W, int precision)
+{
+    for (unsigned i = 0; i < n_views; ++i)
+        if (views[i].e.W != W) rebuild_blocks(&views[i].e, W);
+    /* continuation regions: a quarter of each view's rays (at least 256) */
+    uint64_t cap_total = 0;
+    for (unsigned i = 0; i < n_views; ++i) {
+        uint32_t cap = views[i].n_rays / 4u;
+        if (cap < 256u) cap = 256u;
+        views[i].cont_base = (uint32_t)cap_total;
+        views[i].cont_cap = cap;
+        cap_total += cap;
+    }
+    if (cap_total > 0xFFFFFFFFull)
+        return fail(ctx, LYAP_ERR_UNSUPPORTED, "too many rays in one launch");
+    int rc = ensure_buffer(ctx, &ctx->cont_buf, &ctx->cont_buf_bytes, 2 * cap_total * sizeof(ContEntry));
+    if (rc) return rc;
+    rc = ensure_buffer(ctx, &ctx->cont_counts, &ctx->cont_counts_bytes, 3 * n_views * sizeof(uint32_t));
+    if (rc) return rc;
+    rc = upload_views(ctx, views, n_views);
     if (rc) return rc;
     rc = launch_zero_u32(ctx, ctx->counters, 8);
     if (rc) return rc;
-    RenderArgs a{};
-    a.views = (const ViewDesc *)ctx->view_table;
-    a.n_views = n_views;
-    a.total_tiles = total_tiles;
-    a.tile_rays = tile_rays;
-    a.tile_counter = ctx->counters + 4;
-    a.eval_counter = (unsigned long long *)(ctx->counters + 2);
+    rc = launch_zero_u32(ctx, (uint32_t *)ctx->cont_counts, 3 * n_views);
+    if (rc) return rc;
+    ContEntry *bufA = (ContEntry *)ctx->cont_buf;
+    ContEntry *bufB = bufA + cap_total;
+    uint32_t *counts = (uint32_t *)ctx->cont_counts;
+    bool margin = false;
+    for (unsigned i = 0; i < n_views; ++i)
+        if (views[i].margin > 0.0f && views[i].refine == LYAP_REFINE_BISECT) margin = true;
     const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
-    uint32_t blocks = cus * waves_per_cu / 4u;
-    const uint32_t needed = (total_tiles * tile_rays + 255u) / 256u;
-    if (blocks > needed) blocks = needed;
-    if (blocks == 0) blocks = 1;
-    if (precision == LYAP_PREC_PLAIN)
-        hipLaunchKernelGGL(render_persistent<LYAP_PREC_PLAIN>, dim3(blocks), dim3(256), 0, ctx->stream, a);
-    else if (precision == LYAP_PREC_PRODUCT)
-        hipLaunchKernelGGL(render_persistent<LYAP_PREC_PRODUCT>, dim3(blocks), dim3(256), 0, ctx->stream, a);
-    else
-        hipLaunchKernelGGL(render_persistent<LYAP_PREC_FP64>, dim3(blocks), dim3(256), 0, ctx->stream, a);
-    LYAP_HIP(ctx, hipGetLastError());
+    const uint64_t needed0 = ((uint64_t)total_tiles * plan.tile_rays * plan.lanes_per_ray + 255u) / 256u;
+    /* level l reads buffer/counts l-1 and writes l (buffers alternate) */
+    ContEntry *out_buf[4] = {bufA, bufB, bufA, nullptr};
+    const ContEntry *in_buf[4] = {nullptr, bufA, bufB, bufA};
+    for (uint32_t level = 0; level < 4; ++level) {
+        RenderArgs a{};
+        a.n_views = n_views;
+        a.total_tiles = total_tiles;
+        a.tile_rays = plan.tile_rays;
+        a.lanes_per_ray = plan.lanes_per_ray;
+        a.tile_counter = ctx->counters + 4 + level;
+        a.eval_counter = (unsigned long long *)(ctx->counters + 2);
+        a.wave_log = (unsigned long long *)ctx->debug_wave_log;
+        a.level = level;
+        a.budget = kLevelBudget[level];
+        a.resident_lanes = plan.resident_lanes;
+        a.cont_in = in_buf[level];
+        a.in_counts = level ? counts + (level - 1) * n_views : nullptr;
+        a.cont_out = out_buf[level];
+        a.out_counts = level < 3 ? counts + level * n_views : nullptr;
+        uint32_t blocks = plan.blocks;
+        if (level == 0 && blocks > needed0) blocks = (uint32_t)needed0;
+        if (blocks == 0) blocks = 1;
+        if (blocks > cus * 8u) blocks = cus * 8u;
+#define LYAP_LAUNCH(WW, PP, MM)                                                       \
+    hipLaunchKernelGGL((render_persistent<WW, PP, MM>), dim3(blocks), dim3(256), 0, ctx->stream, \
+                       (const ViewDesc *)ctx->view_table, a)
+        if (W == 16) {
+            if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(16, LYAP_PREC_PLAIN, false);
+            else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, true);
+            else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, false);
+            else LYAP_LAUNCH(16, LYAP_PREC_FP64, false);
+        } else {
+            if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(32, LYAP_PREC_PLAIN, false);
+            else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(32, LYAP_PREC_PRODUCT, true);
+            else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH(32, LYAP_PREC_PRODUCT, false);
+            else LYAP_LAUNCH(32, LYAP_PREC_FP64, false);
+        }
+#undef LYAP_LAUNCH
+        LYAP_HIP(ctx, hipGetLastError());
+    }
     return LYAP_OK;
 }
 
@@ -629,17 +1163,29 @@ int launch_render(lyap_ctx *ctx, LyapRGBA *rgba, LyapPoint *points,
                   unsigned h, float *depth, float *lambda, uint32_t *evals,
                   const LyapRenderMode &mode)
 {
-    uint32_t tile_rays = ctx->tuned.tile_rays ? ctx->tuned.tile_rays : 256u;
-    uint32_t waves = ctx->tuned.waves_per_cu ? ctx->tuned.waves_per_cu : 32u;
+    const uint64_t rays = 64ull * ((w + 7) / 8) * ((h + 7) / 8);
+    const uint32_t W = e.L <= 16 ? 16u : 32u;
+    const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
+    const LaunchPlan plan = plan_launch(ctx, rays, mode.lanes_per_ray, W, mode.precision, margin);
     ViewDesc v;
     fill_view(v, e, cam, prm, w, h, rgba, points, depth, lambda, evals, lights,
-              num_lights, mode, tile_rays, 0);
-    return launch_views(ctx, &v, 1, v.n_tiles, tile_rays, waves, mode.precision);
+              num_lights, mode, plan.tile_rays, 0);
+    return launch_views(ctx, &v, 1, v.n_tiles, plan, W, mode.precision);
 }
 
 } /* namespace lyap */
 
 using namespace lyap;
+
+/* Diagnostic: device buffer of 4 x uint64 per wave that the next render
+ * launches fill with {rounds, rays started, start clock, end clock}; NULL
+ * turns it off. */
+extern "C" int lyap_debug_wave_log(lyap_ctx *ctx, void *d_log)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    ctx->debug_wave_log = d_log;
+    return LYAP_OK;
+}
 
 extern "C" int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_points,
                                 const LyapCam *cam, const LyapParams *prm,

@@ -28,9 +28,10 @@
  *    of leaving the caller's buffer unwritten (kernel.cu:262-264, 361-363);
  *  - to_rgba saturates to [0, 255] (NaN -> 0) where the reference's
  *    float->unsigned char conversion is undefined;
- *  - the refinement walk stops after 64 steps and the march after 2^20
- *    samples (the reference loops forever when refine <= 0 or the step is
- *    not positive);
+ *  - the refinement walk stops after 64 steps, the march after 2^20
+ *    samples, and a march step that does not change t in float (a grazing
+ *    ray whose t1 - t0 is near 0, or depth = inf) ends the ray as a miss:
+ *    the reference loops forever in these cases;
  *  - an optional bisection refinement and absolute normal offset (the
  *    configured render method) next to the reference walk.
  *
@@ -297,6 +298,8 @@ RayOut raymarch(const LyapCam &cam, const LyapParams &prm, const Lambda &lam,
     const float Ndt = dt / prm.nearMultiplier;
     bool near = false;
     float last_step = dt;   /* increment that led to the current sample */
+    float tb = t0, dtb = dt;  /* bisection method: t = tb + m * dtb */
+    uint32_t m = 0;
 
     auto note = [&](double l) {
         const double m = std::fabs(l - (double)prm.opaqueThreshold);
@@ -315,9 +318,23 @@ RayOut raymarch(const LyapCam &cam, const LyapParams &prm, const Lambda &lam,
             else jit = (float)(1.0 + (double)(jit * prm.jitter));
             if (std::isfinite(jit)) step = dt * jit;
         }
-        t += step;
-        if (bisect) P = add(C, mul(V, t));
-        else P = add(P, mul(V, step));
+        /* A step too small to change t (a grazing ray with t1 - t0 near 0,
+         * or depth = inf) would never leave the box: the ray ends. The
+         * bisection method samples t = tb + m*dtb directly, rebased when the
+         * step length changes; the reference walk adds the step (and V*step
+         * to P) each time. */
+        if (bisect) {
+            if (step != dtb) { tb = t; dtb = step; m = 0; }
+            const float tn = tb + (float)(m + 1u) * dtb;
+            if (!(tn > t)) return o;
+            t = tn;
+            ++m;
+            P = add(C, mul(V, t));
+        } else {
+            if (!(t + step > t)) return o;
+            t += step;
+            P = add(P, mul(V, step));
+        }
         last_step = step;
         if (t > t1) return o;
         if (index + 1 >= kMaxSamples) return o;

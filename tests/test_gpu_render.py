@@ -84,6 +84,15 @@ def test_fp64_bisect_matches_oracle(ctx):
     assert_exact(got, ref)
 
 
+def test_zero_step_matches_oracle(ctx):
+    cam, prm, seq, lights, n_lights = W.render_config(32, 32)
+    prm.depth = float("inf")
+    for mode in [W.c2_mode(precision=lyap.PREC_FP64), lyap.render_mode(precision=lyap.PREC_FP64)]:
+        got = gpu_render(ctx, cam, prm, seq, lights, n_lights, 32, 32, mode)
+        ref = O.render(cam, prm, seq, lights, n_lights, 32, 32, mode=mode, lam=O.F64)
+        assert_exact(got, ref)
+
+
 def step_of(cam, prm, w, h, ys, xs):
     dt = np.empty(len(xs))
     for i, (x, y) in enumerate(zip(xs, ys)):
@@ -92,8 +101,15 @@ def step_of(cam, prm, w, h, ys, xs):
     return dt
 
 
+@pytest.fixture(scope="module")
+def c2_oracle():
+    w = h = 512
+    cam, prm, seq, lights, n_lights = W.render_config(w, h)
+    return O.render(cam, prm, seq, lights, n_lights, w, h, mode=W.c2_mode(), lam=O.F64)
+
+
 @pytest.mark.parametrize("prec", [lyap.PREC_FP64, lyap.PREC_PRODUCT, lyap.PREC_PLAIN])
-def test_c2_depth_within_one_step(ctx, prec):
+def test_c2_depth_within_one_step(ctx, prec, c2_oracle):
     """FP64: every sampled ray agrees with the oracle (hit, and depth within
     one step). FP32 forms: a finite-window lambda near a chaotic transient
     is a different realisation in FP32 than in FP64, and a ray samples ~26
@@ -102,8 +118,8 @@ def test_c2_depth_within_one_step(ctx, prec):
     w = h = 512
     cam, prm, seq, lights, n_lights = W.render_config(w, h)
     got = gpu_render(ctx, cam, prm, seq, lights, n_lights, w, h, W.c2_mode(precision=prec))
-    rows = (3, 8)
-    ref = O.render(cam, prm, seq, lights, n_lights, w, h, mode=W.c2_mode(), lam=O.F64, rows=rows)
+    rows = (0, 1)
+    ref = c2_oracle
     ys = np.arange(rows[0], h, rows[1])
     g_d = got["depth"][ys]
     r_d = ref["depth"][ys]

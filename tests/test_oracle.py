@@ -210,6 +210,18 @@ def test_slab_clip_closed_form():
     assert np.allclose(V, S0 / np.linalg.norm(S0) / M, atol=1e-6)
 
 
+def test_zero_step_ends_the_ray():
+    """depth = inf gives a zero step: a transparent first sample cannot
+    advance, so the ray is a miss after one evaluation (the reference would
+    loop forever)."""
+    cam, prm, lights, n, mode = analytic_scene(4, 4)
+    prm.depth = float("inf")
+    prm.opaqueThreshold = -5.0                 # everything transparent
+    out = O.render(cam, prm, "A", lights, n, 4, 4, mode=mode)
+    assert np.all(np.isnan(out["depth"]))
+    assert np.all(out["evals"][~np.isnan(out["margin"])] == 1)
+
+
 def test_miss_record():
     cam, prm, lights, n, mode = analytic_scene(4, 4)
     cam = lyap.cam_look_at((4.0, 9.0, 9.0), (4.0, 12.0, 12.0), 0.2, 4, 4)   # looks away
