@@ -1,17 +1,29 @@
 #!/usr/bin/env python3
-"""Throughput of the Lyapunov raymarch on one MI355X.
+"""Throughput of the Lyapunov path on one MI355X.
 
-Workload (one step = one frame): the plain render, a 512x512 image of the
-[2,4]^3 Lyapunov box seen from distance 4 along a seeded unit quaternion,
-45 degree field of view, sequence AABAB, 100 settle + 300 accumulated
-iterations per lambda, up to 500 uniform steps per ray, surface at the first
-lambda <= 0, 8 bisection steps, normal from six samples at h = 1e-3, Phong
-shading, outputs RGBA8 + depth + lambda + LyapPoint per pixel.
+Headline workload (one step = one frame): the plain render, a 512x512 image
+of the [2,4]^3 Lyapunov box seen from distance 4 along a seeded unit
+quaternion, 45 degree field of view, sequence AABAB, 100 settle + 300
+accumulated iterations per lambda, up to 500 uniform steps per ray, surface
+at the first lambda <= 0, 8 bisection steps, normal from six samples at
+h = 1e-3, Phong shading, outputs RGBA8 + depth + lambda + LyapPoint per pixel.
+`value` is rays/s of that frame loop with all buffers resident in HBM.
 
-The JSON line reports rays/s (value) and lambda-evals/s, the roofline of the
-render kernel (algorithmic FLOP per lambda-eval = 3*settle + 6*accum, against
-MI355X's FP32 vector peak) and the oracle's FP32 CPU path timed on a bounded
-sample of rows on this host.
+Also reported (same JSON line, "workloads"):
+  c2  the same frame with FP64 re-evaluation near the threshold during
+      bisection, in exact FP64, and in the plain FP32 per-step-log form;
+  c1  64^3 volume over [2,4]^3, ABBAC, 100 + 400 (plain and product forms);
+  c3  256^3 volume over [2.5,4]^3, ABCABCCA, 200 + 2000: plain per-step-log,
+      product, and product with FP64 re-evaluation, with max/RMS |dlambda|
+      and sign agreement against the FP64 oracle on 20k seeded voxels;
+  c4  64 views of seeded size 128..1024 in one batched launch sequence
+      (camera radius 3..6, sequences of 3-12 letters, 200..2000 iterations),
+      against rendering the same views one request at a time in the plain
+      FP32 per-step-log form.
+The roofline uses algorithmic work: 3*settle + 6*accum FLOP per lambda
+evaluation, against MI355X's FP32 vector peak of 157.3 TFLOP/s. The CPU
+baseline is the oracle's FP32-as-written render of a bounded sample of rows,
+timed on this host.
 """
 from __future__ import annotations
 
@@ -26,28 +38,59 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 vector (packed)
-HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3     # MI355X dense FP32 vector
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--precision", default="product", choices=["product", "plain", "fp64"])
     p.add_argument("--size", type=int, default=512)
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="approximate CPU budget of the baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--workloads", default="c2,c1,c3,c4", help="extra workloads ('' for none)")
+    p.add_argument("--c4-views", type=int, default=64)
     return p.parse_args()
+
+
+def flops(prm):
+    return 3 * prm.settle + 6 * prm.accum
+
+
+class Timer:
+    """HIP events on the library's stream."""
+
+    def __init__(self, torch, ctx):
+        self.torch = torch
+        self.ctx = ctx
+        self.stream = torch.cuda.ExternalStream(ctx.stream)
+
+    def run(self, fn, reps):
+        t = self.torch
+        fn()
+        self.ctx.sync()
+        s = t.cuda.Event(enable_timing=True)
+        e = t.cuda.Event(enable_timing=True)
+        s.record(self.stream)
+        for _ in range(reps):
+            fn()
+        e.record(self.stream)
+        self.ctx.sync()
+        t.cuda.synchronize()
+        return s.elapsed_time(e) / reps
 
 
 def cpu_baseline(cam, prm, seq, lights, n_lights, w, h, mode, budget_s):
     """The oracle's FP32 as-written render on every k-th row, timed here."""
     from oracle import oracle as O
     threads = O.threads()
-    # probe one row to size the sample
     t = time.perf_counter()
     O.render(cam, prm, seq, lights, n_lights, w, h, mode=mode, lam=O.F32W, rows=(h // 2, h), nthreads=threads)
     one_row = max(time.perf_counter() - t, 1e-4)
@@ -65,6 +108,100 @@ def cpu_baseline(cam, prm, seq, lights, n_lights, w, h, mode, budget_s):
     }
 
 
+def volume_stats(got, ref, h1, h2):
+    stable = (ref < -1e-3) & (np.abs(h1 - h2) < 1e-4)
+    d = np.abs(got[stable].astype(np.float64) - ref[stable])
+    d = np.nan_to_num(d, nan=9.0)
+    sign = np.mean((np.sign(got) == np.sign(ref)) | (np.isnan(got) & np.isnan(ref)))
+    return {"stable_fraction": float(stable.mean()), "max_abs": float(d.max()) if d.size else 0.0,
+            "rms": float(np.sqrt(np.mean(d ** 2))) if d.size else 0.0,
+            "within_1e-3": float(np.mean(d <= 1e-3)) if d.size else 1.0, "sign_agreement": float(sign)}
+
+
+def bench_volumes(lyap, timer, ctx, which):
+    from tests import workloads as W
+    from tests.gpu_util import empty, to_numpy
+    out = {}
+    if "c1" in which:
+        prm, seq, lo, hi, n = W.volume_config("c1")
+        buf = empty(4 * n ** 3)
+        res = {}
+        for name, mode in [("plain", lyap.volume_mode(lyap.PREC_PLAIN)),
+                           ("product", lyap.volume_mode(lyap.PREC_PRODUCT))]:
+            ms = timer.run(lambda: ctx.calc_volume(buf.data_ptr(), prm, seq, lo, hi, n, mode), 20)
+            res[name] = {"ms": ms, "lambda_evals_per_s": n ** 3 / (ms * 1e-3),
+                         "tflops": n ** 3 * flops(prm) / (ms * 1e-3) / 1e12}
+        out["c1_volume_64"] = res
+        log("c1 done")
+    if "c3" in which:
+        from oracle import oracle as O
+        prm, seq, lo, hi, n = W.volume_config("c3")
+        buf = empty(4 * n ** 3)
+        rng = np.random.default_rng(3)
+        idx = rng.integers(0, n, size=(20000, 3))
+        c = O.grid_coords(lo[0], hi[0], n)
+        xyz = np.stack([c[idx[:, 0]], c[idx[:, 1]], c[idx[:, 2]]], -1)
+        ref = O.lyap_points(xyz, prm.d, prm.settle, prm.accum, seq)
+        h1, h2 = O.lyap_halves(xyz, prm.d, prm.settle, prm.accum, prm.accum // 2, seq)
+        res = {}
+        plain_ms = None
+        for name, mode in [("plain", lyap.volume_mode(lyap.PREC_PLAIN)),
+                           ("product", lyap.volume_mode(lyap.PREC_PRODUCT)),
+                           ("product_fp64_margin", lyap.volume_mode(**W.C3_MODE)),
+                           ("product_fp64_strict", lyap.volume_mode(**W.C3_STRICT))]:
+            ms = timer.run(lambda: ctx.calc_volume(buf.data_ptr(), prm, seq, lo, hi, n, mode), 5)
+            re = ctx.last_reevaluated()
+            got = to_numpy(buf, np.float32, (n, n, n))[idx[:, 2], idx[:, 1], idx[:, 0]]
+            r = {"ms": ms, "lambda_evals_per_s": n ** 3 / (ms * 1e-3),
+                 "tflops": n ** 3 * flops(prm) / (ms * 1e-3) / 1e12,
+                 "roofline_frac": n ** 3 * flops(prm) / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                 "fp64_reevaluated_fraction": re / n ** 3, "vs_fp64_oracle": volume_stats(got, ref, h1, h2)}
+            if name == "plain":
+                plain_ms = ms
+            else:
+                r["speedup_vs_plain"] = plain_ms / ms
+            res[name] = r
+        out["c3_volume_256"] = res
+        log("c3 done")
+    return out
+
+
+def bench_batch(lyap, timer, ctx, n_views):
+    import torch
+    from tests import workloads as W
+    from tests.gpu_util import RenderBuffers, device_lights
+    views, lights, n_lights = W.batch_views(n_views, seed=4)
+    ld = device_lights(lights, n_lights)
+    bufs = [RenderBuffers(v["w"], v["h"]) for v in views]
+    api = [lyap.make_view(v["cam"], v["prm"], v["seq"], v["w"], v["h"], b.rgba.data_ptr(), b.points.data_ptr(),
+                          b.depth.data_ptr(), b.lam.data_ptr(), b.evals.data_ptr(), ld.data_ptr(), n_lights)
+           for v, b in zip(views, bufs)]
+    rays = sum(v["w"] * v["h"] for v in views)
+    mode = W.c2_mode(fp64_margin=0.0)
+    ms = timer.run(lambda: ctx.render_batch(api, mode), 3)
+    evals = ctx.last_evals()
+    flop = 0
+    # algorithmic FLOP: per-view evaluations x per-view FLOP per evaluation
+    for v, b in zip(views, bufs):
+        flop += int(b.evals.cpu().numpy().view(np.uint32).astype(np.int64).sum()) * flops(v["prm"])
+    frame = ctx.frame_capture(api, mode)
+    graph_ms = timer.run(lambda: frame.replay(), 3)
+    frame.close()
+    # the same views one request at a time, plain FP32 per-step log
+    plain_mode = W.c2_mode(precision=lyap.PREC_PLAIN, fp64_margin=0.0)
+
+    def plain_all():
+        for v, b in zip(views, bufs):
+            b.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, plain_mode)
+    plain_ms = timer.run(plain_all, 1)
+    log("c4 done")
+    return {"views": n_views, "rays": rays, "ms": ms, "graph_replay_ms": graph_ms,
+            "rays_per_s": rays / (ms * 1e-3), "lambda_evals_per_s": evals / (ms * 1e-3),
+            "tflops": flop / (ms * 1e-3) / 1e12, "roofline_frac": flop / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+            "plain_per_request_ms": plain_ms, "plain_per_request_rays_per_s": rays / (plain_ms * 1e-3),
+            "speedup_vs_plain_per_request": plain_ms / ms}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -79,40 +216,36 @@ def main():
     prec = {"product": lyap.PREC_PRODUCT, "plain": lyap.PREC_PLAIN, "fp64": lyap.PREC_FP64}[args.precision]
     w = h = args.size
     cam, prm, seq, lights, n_lights = W.render_config(w, h)
-    mode = W.c2_mode(precision=prec)
+    mode = W.c2_mode(precision=prec, fp64_margin=0.0)
     ctx = lyap.Context(0)
+    timer = Timer(torch, ctx)
     bufs = RenderBuffers(w, h)
     ld = device_lights(lights, n_lights)
-    stream = torch.cuda.ExternalStream(ctx.stream)
 
-    print(f"bench: warmup {args.warmup} frames", file=sys.stderr, flush=True)
+    log(f"warmup {args.warmup} frames")
     for _ in range(args.warmup):
         bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
     ctx.sync()
-    print("bench: warmup done", file=sys.stderr, flush=True)
     evals_per_frame = ctx.last_evals()
 
     torch.cuda.synchronize()
-    ctx.sync()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     t0 = time.perf_counter()
     for i in range(args.steps):
-        starts[i].record(stream)
+        starts[i].record(timer.stream)
         bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
-        ends[i].record(stream)
+        ends[i].record(timer.stream)
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    print(f"bench: timed {args.steps} frames in {wall:.3f} s", file=sys.stderr, flush=True)
-    per = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])   # ms
+    log(f"timed {args.steps} frames in {wall:.3f} s")
+    per = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])   # ms, on the library stream
     ms = float(wall * 1000.0 / args.steps)
 
     rays = w * h
-    flop_per_eval = 3 * prm.settle + 6 * prm.accum
-    kernel_ms = float(per.mean())
-    achieved = evals_per_frame * flop_per_eval / (kernel_ms * 1e-3) / 1e12
-    out_bytes = rays * (4 + 36 + 4 + 4 + 4)
+    frame_ms = float(per.mean())
+    achieved = evals_per_frame * flops(prm) / (frame_ms * 1e-3) / 1e12
     line = {
         "metric": "lambda-evals/s & rays/s on 1 MI355X; p50/p99 latency, served vs plain",
         "value": rays / (ms * 1e-3),
@@ -128,28 +261,45 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"plain render {w}x{h}, seq {seq}, settle {prm.settle} accum {prm.accum}, "
-                        f"500 uniform steps, 8 bisections, h=1e-3, Phong",
+                        f"500 uniform steps, 8 bisections, h=1e-3, Phong, camera seed 1",
             "precision": args.precision,
-            "camera_seed": 1,
         },
         "lambda_evals_per_s": evals_per_frame / (ms * 1e-3),
         "lambda_evals_per_frame": int(evals_per_frame),
-        "kernel_ms_mean": kernel_ms,
-        "kernel_ms_p50": float(np.percentile(per, 50)),
-        "kernel_ms_p99": float(np.percentile(per, 99)),
+        "frame_ms_events_mean": frame_ms,
+        "frame_ms_p50": float(np.percentile(per, 50)),
+        "frame_ms_p99": float(np.percentile(per, 99)),
         "roofline": {
-            "bound": "valu-fp32",
+            "bound": "valu",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
-            "flop_per_lambda_eval": flop_per_eval,
-            "output_bytes_per_frame": out_bytes,
+            "flop_per_lambda_eval": flops(prm),
+            "note": "render kernel (four level launches per frame), algorithmic FLOP per frame / frame time",
         },
     }
+    which = [x for x in args.workloads.split(",") if x]
+    if which:
+        extra = {}
+        if "c2" in which:
+            # the same frame in the other arithmetic forms
+            c2 = {}
+            for name, m in [("fp32_product_fp64_bisect_margin", W.c2_mode(fp64_margin=0.01)),
+                            ("fp64_exact", W.c2_mode(precision=lyap.PREC_FP64, fp64_margin=0.0)),
+                            ("fp32_plain", W.c2_mode(precision=lyap.PREC_PLAIN, fp64_margin=0.0))]:
+                fms = timer.run(lambda: bufs.render(ctx, cam, prm, seq, ld, n_lights, m), 10)
+                c2[name] = {"ms": fms, "rays_per_s": rays / (fms * 1e-3),
+                            "lambda_evals_per_s": ctx.last_evals() / (fms * 1e-3)}
+            extra["c2_variants"] = c2
+            log("c2 variants done")
+        extra.update(bench_volumes(lyap, timer, ctx, which))
+        if "c4" in which:
+            extra["c4_batched"] = bench_batch(lyap, timer, ctx, args.c4_views)
+        line["workloads"] = extra
     if not args.no_cpu:
-        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(cam, prm, seq, lights, n_lights, w, h, mode, args.cpu_seconds)
     ctx.close()
     print(json.dumps(line), flush=True)

@@ -439,3 +439,65 @@ class Context:
         c = C.c_uint64()
         _check(self.lib.lyap_last_evals(self.h, C.byref(c)), self.h, "lyap_last_evals")
         return c.value
+
+    # ---- batched views -------------------------------------------------
+    def render_batch(self, views, mode: Optional[RenderMode] = None, tuning: Optional[Tuning] = None):
+        arr = (View * len(views))(*views)
+        rc = self.lib.lyap_render_batch(self.h, arr, len(views), C.byref(mode) if mode is not None else None,
+                                        C.byref(tuning) if tuning is not None else None)
+        _check(rc, self.h, "lyap_render_batch")
+
+    def autotune(self, views, mode: Optional[RenderMode] = None) -> Tuning:
+        arr = (View * len(views))(*views)
+        best = Tuning()
+        rc = self.lib.lyap_autotune(self.h, arr, len(views), C.byref(mode) if mode is not None else None,
+                                    C.byref(best))
+        _check(rc, self.h, "lyap_autotune")
+        return best
+
+    def frame_capture(self, views, mode: Optional[RenderMode] = None, tuning: Optional[Tuning] = None):
+        arr = (View * len(views))(*views)
+        f = C.c_void_p()
+        rc = self.lib.lyap_frame_capture(self.h, arr, len(views), C.byref(mode) if mode is not None else None,
+                                         C.byref(tuning) if tuning is not None else None, C.byref(f))
+        _check(rc, self.h, "lyap_frame_capture")
+        return Frame(self, f)
+
+
+class Frame:
+    """A captured frame (lyap_frame): replay() relaunches its graph."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    def replay(self):
+        _check(self.ctx.lib.lyap_frame_replay(self.ctx.h, self.h), self.ctx.h, "lyap_frame_replay")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.lyap_frame_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_view(cam: LyapCam, prm: LyapParams, seq, w: int, h: int, rgba: int, points: int = 0,
+              depth: int = 0, lam: int = 0, evals: int = 0, lights: int = 0, num_lights: int = 0) -> View:
+    """A View for the batched entries; device outputs as integer addresses."""
+    v = View()
+    v.cam = cam
+    v.prm = prm
+    s = list(seq_array(seq))
+    if len(s) > MAX_SEQ + 1:
+        raise LyapError(ERR_BAD_SEQUENCE, "sequence longer than %d" % MAX_SEQ)
+    for i, x in enumerate(s):
+        v.seq[i] = x
+    v.width, v.height = w, h
+    v.rgba, v.points, v.depth, v.lambda_, v.evals = rgba, points or None, depth or None, lam or None, evals or None
+    v.lights, v.num_lights = lights or None, num_lights
+    return v

@@ -140,9 +140,9 @@ void lyap_destroy(lyap_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_render_resources(ctx);
     for (auto &kv : ctx->allocations) (void)hipFree(kv.first);
     ctx->allocations.clear();
-    if (ctx->view_staging) (void)hipHostFree(ctx->view_staging);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -244,7 +244,8 @@ int lyap_last_evals(lyap_ctx *ctx, uint64_t *count)
 {
     if (!ctx || !count) return LYAP_ERR_INVALID_ARGUMENT;
     unsigned long long c = 0;
-    LYAP_HIP(ctx, hipMemcpyAsync(&c, ctx->counters + 2, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    const uint32_t *src = ctx->last_counters ? ctx->last_counters : ctx->counters;
+    LYAP_HIP(ctx, hipMemcpyAsync(&c, src + 2, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *count = c;
     return LYAP_OK;

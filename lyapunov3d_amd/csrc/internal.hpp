@@ -26,15 +26,9 @@ struct lyap_ctx {
     uint32_t *reeval_list = nullptr;   /* voxel indices for the FP64 pass */
     size_t reeval_cap = 0;             /* entries */
 
-    /* device view table of the batched render */
-    void *view_table = nullptr;
-    size_t view_table_bytes = 0;
-    void *view_staging = nullptr;      /* pinned host copy */
-    size_t view_staging_bytes = 0;
-    void *cont_buf = nullptr;          /* rays handed between render launches */
-    size_t cont_buf_bytes = 0;
-    void *cont_counts = nullptr;       /* per-view counts of handed-over rays */
-    size_t cont_counts_bytes = 0;
+    /* the render path's launch resources (render.hip) */
+    void *render_res = nullptr;
+    uint32_t *last_counters = nullptr; /* counters of the last render launch */
 
     LyapTuning tuned{};                /* autotuner's choice, zero = unset */
     void *debug_wave_log = nullptr;    /* per-wave render record, when set */
@@ -58,6 +52,8 @@ int launch_volume(lyap_ctx *ctx, float *out, const EvalDesc &e,
                   const float lo[3], const float hi[3], unsigned nx,
                   unsigned ny, unsigned nz, const float *xyz, size_t npoints,
                   const LyapVolumeMode &mode);
+
+void free_render_resources(lyap_ctx *ctx);
 
 int launch_render(lyap_ctx *ctx, LyapRGBA *rgba, LyapPoint *points,
                   const LyapCam &cam, const LyapParams &prm, const EvalDesc &e,

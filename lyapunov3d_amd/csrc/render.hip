@@ -21,8 +21,10 @@
  * Ray order inside a view is 8x8 pixel blocks, so the 64 rays a wave starts
  * together are neighbours with similar march lengths.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "internal.hpp"
 
@@ -942,8 +944,7 @@ static void fill_view(ViewDesc &v, const EvalDesc &e, const LyapCam &cam,
                       const LyapParams &prm, unsigned w, unsigned h,
                       LyapRGBA *rgba, LyapPoint *points, float *depth,
                       float *lambda, uint32_t *evals, const LyapLight *lights,
-                      unsigned num_lights, const LyapRenderMode &mode,
-                      uint32_t tile_rays, uint32_t tile_begin)
+                      unsigned num_lights, const LyapRenderMode &mode)
 {
     std::memset(&v, 0, sizeof(v));
     v.e = e;
@@ -953,8 +954,6 @@ static void fill_view(ViewDesc &v, const EvalDesc &e, const LyapCam &cam,
     v.h = h;
     v.blocks_x = (w + 7) / 8;
     v.n_rays = 64u * v.blocks_x * ((h + 7) / 8);
-    v.tile_begin = tile_begin;
-    v.n_tiles = (v.n_rays + tile_rays - 1) / tile_rays;
     v.rgba = rgba;
     v.points = points;
     v.depth = depth;
@@ -969,33 +968,7 @@ static void fill_view(ViewDesc &v, const EvalDesc &e, const LyapCam &cam,
     v.max_samples = 1u << 20;
 }
 
-static int upload_views(lyap_ctx *ctx, const ViewDesc *views, unsigned n)
-{
-    const size_t bytes = sizeof(ViewDesc) * n;
-    if (bytes > ctx->view_table_bytes) {
-        LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        ctx_free(ctx, ctx->view_table);
-        ctx->view_table = nullptr;
-        ctx->view_table_bytes = 0;
-        int rc = ctx_alloc(ctx, &ctx->view_table, bytes);
-        if (rc) return rc;
-        ctx->view_table_bytes = bytes;
-    }
-    if (bytes > ctx->view_staging_bytes) {
-        LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->view_staging) (void)hipHostFree(ctx->view_staging);
-        ctx->view_staging = nullptr;
-        ctx->view_staging_bytes = 0;
-        LYAP_HIP(ctx, hipHostMalloc(&ctx->view_staging, bytes, hipHostMallocDefault));
-        ctx->view_staging_bytes = bytes;
-    }
-    /* the staging copy of the previous launch must have been consumed */
-    LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    std::memcpy(ctx->view_staging, views, bytes);
-    LYAP_HIP(ctx, hipMemcpyAsync(ctx->view_table, ctx->view_staging, bytes,
-                                 hipMemcpyHostToDevice, ctx->stream));
-    return LYAP_OK;
-}
+/* ---- launch planning ----------------------------------------------------- */
 
 /* Lanes per ray while the queue still has rays: the power of two that lets
  * the launch's rays fill the resident lanes once, at least 4 (a group of 4
@@ -1049,28 +1022,45 @@ static uint32_t kernel_blocks_per_cu(uint32_t W, int precision, bool margin)
     return resident_blocks_per_cu<32, LYAP_PREC_FP64, false>();
 }
 
-static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, uint32_t lanes_hint,
+static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, const LyapTuning &tune,
                               uint32_t W, int precision, bool margin)
 {
     LaunchPlan p;
     const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
     uint32_t per_cu = kernel_blocks_per_cu(W, precision, margin);
-    if (ctx->tuned.waves_per_cu) per_cu = (ctx->tuned.waves_per_cu + 3u) / 4u;
+    if (tune.waves_per_cu && (tune.waves_per_cu + 3u) / 4u < per_cu) per_cu = (tune.waves_per_cu + 3u) / 4u;
     p.blocks = cus * per_cu;
     p.resident_lanes = p.blocks * 256u;
-    p.lanes_per_ray = lanes_hint ? lanes_hint
-                      : (ctx->tuned.lanes_per_ray ? ctx->tuned.lanes_per_ray
-                                                  : default_lanes_per_ray(p.resident_lanes, rays));
+    p.lanes_per_ray = tune.lanes_per_ray ? tune.lanes_per_ray
+                                         : default_lanes_per_ray(p.resident_lanes, rays);
     if (p.lanes_per_ray > 64u) p.lanes_per_ray = 64u;
     if (p.lanes_per_ray == 0) p.lanes_per_ray = 1;
-    p.tile_rays = ctx->tuned.tile_rays ? ctx->tuned.tile_rays : 64u / p.lanes_per_ray;
+    p.tile_rays = tune.tile_rays ? tune.tile_rays : 64u / p.lanes_per_ray;
+    if (p.tile_rays == 0) p.tile_rays = 1;
     return p;
 }
 
-static int ensure_buffer(lyap_ctx *ctx, void **buf, size_t *cap, size_t bytes)
+/* ---- launch resources ---------------------------------------------------- */
+
+/* Device and pinned memory one launch sequence uses. The context owns one
+ * set for its direct calls; a captured frame owns its own, so a replay never
+ * sees memory that a later call reallocated. */
+struct RenderResources {
+    void *view_table = nullptr;      /* device ViewDesc[] */
+    size_t view_table_bytes = 0;
+    void *staging = nullptr;         /* pinned host copy of the table */
+    size_t staging_bytes = 0;
+    uint32_t *counters = nullptr;    /* [2..3] evaluations, [4..7] tile heads */
+    void *cont_buf = nullptr;        /* rays handed between levels */
+    size_t cont_buf_bytes = 0;
+    void *cont_counts = nullptr;     /* per view and level */
+    size_t cont_counts_bytes = 0;
+};
+
+static int ensure_buffer(lyap_ctx *ctx, void **buf, size_t *cap, size_t bytes, bool may_sync)
 {
     if (bytes <= *cap) return LYAP_OK;
-    LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (may_sync) LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx_free(ctx, *buf);
     *buf = nullptr;
     *cap = 0;
@@ -1080,14 +1070,31 @@ static int ensure_buffer(lyap_ctx *ctx, void **buf, size_t *cap, size_t bytes)
     return LYAP_OK;
 }
 
-int launch_views(lyap_ctx *ctx, ViewDesc *views, unsigned n_views,
-                 uint32_t total_tiles, const LaunchPlan &plan, uint32_t W, int precision)
+static void release_resources(lyap_ctx *ctx, RenderResources &r)
 {
-    for (unsigned i = 0; i < n_views; ++i)
-        if (views[i].e.W != W) rebuild_blocks(&views[i].e, W);
-    /* continuation regions: a quarter of each view's rays (at least 256) */
+    ctx_free(ctx, r.view_table);
+    ctx_free(ctx, r.cont_buf);
+    ctx_free(ctx, r.cont_counts);
+    ctx_free(ctx, r.counters);
+    if (r.staging) (void)hipHostFree(r.staging);
+    r = RenderResources{};
+}
+
+/* Prepares `views` for a launch: tile ranges, continuation regions, the
+ * common coefficient window; sizes the resources and fills the pinned
+ * staging copy. `may_sync` is false while a stream capture is open. */
+static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
+                         unsigned n_views, const LaunchPlan &plan, uint32_t W,
+                         bool may_sync, uint32_t *total_tiles)
+{
+    uint32_t tiles = 0;
     uint64_t cap_total = 0;
     for (unsigned i = 0; i < n_views; ++i) {
+        if (views[i].e.W != W) rebuild_blocks(&views[i].e, W);
+        views[i].tile_begin = tiles;
+        views[i].n_tiles = (views[i].n_rays + plan.tile_rays - 1) / plan.tile_rays;
+        tiles += views[i].n_tiles;
+        /* continuation region: a quarter of the view's rays, at least 256 */
         uint32_t cap = views[i].n_rays / 4u;
         if (cap < 256u) cap = 256u;
         views[i].cont_base = (uint32_t)cap_total;
@@ -1096,25 +1103,57 @@ int launch_views(lyap_ctx *ctx, ViewDesc *views, unsigned n_views,
     }
     if (cap_total > 0xFFFFFFFFull)
         return fail(ctx, LYAP_ERR_UNSUPPORTED, "too many rays in one launch");
-    int rc = ensure_buffer(ctx, &ctx->cont_buf, &ctx->cont_buf_bytes, 2 * cap_total * sizeof(ContEntry));
+    *total_tiles = tiles;
+    const size_t table = sizeof(ViewDesc) * n_views;
+    int rc = ensure_buffer(ctx, &res.cont_buf, &res.cont_buf_bytes, 2 * cap_total * sizeof(ContEntry), may_sync);
     if (rc) return rc;
-    rc = ensure_buffer(ctx, &ctx->cont_counts, &ctx->cont_counts_bytes, 3 * n_views * sizeof(uint32_t));
+    rc = ensure_buffer(ctx, &res.cont_counts, &res.cont_counts_bytes, 3 * n_views * sizeof(uint32_t), may_sync);
     if (rc) return rc;
-    rc = upload_views(ctx, views, n_views);
+    rc = ensure_buffer(ctx, &res.view_table, &res.view_table_bytes, table, may_sync);
     if (rc) return rc;
-    rc = launch_zero_u32(ctx, ctx->counters, 8);
+    if (!res.counters) {
+        void *c = nullptr;
+        rc = ctx_alloc(ctx, &c, 256);
+        if (rc) return rc;
+        res.counters = (uint32_t *)c;
+    }
+    if (table > res.staging_bytes) {
+        if (may_sync) LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (res.staging) (void)hipHostFree(res.staging);
+        res.staging = nullptr;
+        res.staging_bytes = 0;
+        LYAP_HIP(ctx, hipHostMalloc(&res.staging, table, hipHostMallocDefault));
+        res.staging_bytes = table;
+    }
+    /* the previous launch's copy out of the staging buffer must be done */
+    if (may_sync) LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::memcpy(res.staging, views, table);
+    return LYAP_OK;
+}
+
+/* Enqueues the upload and the level launches on the context's stream. No
+ * call here synchronises, so it may be recorded into a graph. */
+static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *views,
+                         unsigned n_views, uint32_t total_tiles, const LaunchPlan &plan,
+                         uint32_t W, int precision)
+{
+    LYAP_HIP(ctx, hipMemcpyAsync(res.view_table, res.staging, sizeof(ViewDesc) * n_views,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    int rc = launch_zero_u32(ctx, res.counters, 8);
     if (rc) return rc;
-    rc = launch_zero_u32(ctx, (uint32_t *)ctx->cont_counts, 3 * n_views);
+    rc = launch_zero_u32(ctx, (uint32_t *)res.cont_counts, 3 * n_views);
     if (rc) return rc;
-    ContEntry *bufA = (ContEntry *)ctx->cont_buf;
+    uint64_t cap_total = 0;
+    for (unsigned i = 0; i < n_views; ++i) cap_total += views[i].cont_cap;
+    ContEntry *bufA = (ContEntry *)res.cont_buf;
     ContEntry *bufB = bufA + cap_total;
-    uint32_t *counts = (uint32_t *)ctx->cont_counts;
+    uint32_t *counts = (uint32_t *)res.cont_counts;
     bool margin = false;
     for (unsigned i = 0; i < n_views; ++i)
         if (views[i].margin > 0.0f && views[i].refine == LYAP_REFINE_BISECT) margin = true;
     const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
     const uint64_t needed0 = ((uint64_t)total_tiles * plan.tile_rays * plan.lanes_per_ray + 255u) / 256u;
-    /* level l reads buffer/counts l-1 and writes l (buffers alternate) */
+    /* level l reads the rays level l-1 handed over (buffers alternate) */
     ContEntry *out_buf[4] = {bufA, bufB, bufA, nullptr};
     const ContEntry *in_buf[4] = {nullptr, bufA, bufB, bufA};
     for (uint32_t level = 0; level < 4; ++level) {
@@ -1123,8 +1162,8 @@ int launch_views(lyap_ctx *ctx, ViewDesc *views, unsigned n_views,
         a.total_tiles = total_tiles;
         a.tile_rays = plan.tile_rays;
         a.lanes_per_ray = plan.lanes_per_ray;
-        a.tile_counter = ctx->counters + 4 + level;
-        a.eval_counter = (unsigned long long *)(ctx->counters + 2);
+        a.tile_counter = res.counters + 4 + level;
+        a.eval_counter = (unsigned long long *)(res.counters + 2);
         a.wave_log = (unsigned long long *)ctx->debug_wave_log;
         a.level = level;
         a.budget = kLevelBudget[level];
@@ -1139,7 +1178,7 @@ int launch_views(lyap_ctx *ctx, ViewDesc *views, unsigned n_views,
         if (blocks > cus * 8u) blocks = cus * 8u;
 #define LYAP_LAUNCH(WW, PP, MM)                                                       \
     hipLaunchKernelGGL((render_persistent<WW, PP, MM>), dim3(blocks), dim3(256), 0, ctx->stream, \
-                       (const ViewDesc *)ctx->view_table, a)
+                       (const ViewDesc *)res.view_table, a)
         if (W == 16) {
             if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(16, LYAP_PREC_PLAIN, false);
             else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, true);
@@ -1154,7 +1193,109 @@ int launch_views(lyap_ctx *ctx, ViewDesc *views, unsigned n_views,
 #undef LYAP_LAUNCH
         LYAP_HIP(ctx, hipGetLastError());
     }
+    ctx->last_counters = res.counters;
     return LYAP_OK;
+}
+
+static RenderResources &ctx_resources(lyap_ctx *ctx)
+{
+    if (!ctx->render_res) ctx->render_res = new RenderResources();
+    return *static_cast<RenderResources *>(ctx->render_res);
+}
+
+void free_render_resources(lyap_ctx *ctx)
+{
+    if (!ctx->render_res) return;
+    RenderResources *r = static_cast<RenderResources *>(ctx->render_res);
+    release_resources(ctx, *r);
+    delete r;
+    ctx->render_res = nullptr;
+}
+
+/* ---- validation shared by the entries ------------------------------------ */
+
+static int check_mode(lyap_ctx *ctx, const LyapRenderMode *mode_in, LyapRenderMode &mode)
+{
+    mode = LyapRenderMode{};
+    mode.refine = LYAP_REFINE_WALK;
+    mode.precision = LYAP_PREC_PLAIN;
+    if (mode_in) mode = *mode_in;
+    if (mode.refine != LYAP_REFINE_WALK && mode.refine != LYAP_REFINE_BISECT)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown refine mode %d", mode.refine);
+    if (mode.precision < LYAP_PREC_PLAIN || mode.precision > LYAP_PREC_FP64)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown precision %d", mode.precision);
+    if (mode.bisect_steps < 0 || mode.bisect_steps > 64)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "bisect_steps outside 0..64");
+    if (mode.lanes_per_ray > 64)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "lanes_per_ray above 64");
+    return LYAP_OK;
+}
+
+static int view_from_api(lyap_ctx *ctx, ViewDesc &v, const LyapView &in, const LyapRenderMode &mode)
+{
+    if (in.width == 0 || in.height == 0) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "empty view");
+    if ((uint64_t)in.width * in.height > (1ull << 31))
+        return fail(ctx, LYAP_ERR_UNSUPPORTED, "view larger than 2^31 pixels");
+    if (!in.rgba) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null rgba output");
+    if (in.num_lights > LYAP_MAX_LIGHTS) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "more than %d lights", LYAP_MAX_LIGHTS);
+    if (in.num_lights > 0 && !in.lights) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null lights");
+    EvalDesc e;
+    if (!make_eval_desc(&e, in.prm, in.seq, 16))
+        return fail(ctx, LYAP_ERR_BAD_SEQUENCE, "sequence must hold 1..%d letter indices 0..3 before the -1", LYAP_MAX_SEQ);
+    fill_view(v, e, in.cam, in.prm, in.width, in.height, in.rgba, in.points, in.depth,
+              in.lambda, in.evals, in.lights, in.num_lights, mode);
+    return LYAP_OK;
+}
+
+/* Builds the launch of n views: validated descriptors ordered by estimated
+ * cost (largest first, so the end of the queue holds the cheap tiles). */
+static int build_batch(lyap_ctx *ctx, const LyapView *in, unsigned n, const LyapRenderMode &mode,
+                       std::vector<ViewDesc> &views, uint32_t &W)
+{
+    views.resize(n);
+    std::vector<std::pair<double, unsigned>> order(n);
+    uint32_t maxL = 1;
+    for (unsigned i = 0; i < n; ++i) {
+        ViewDesc v;
+        int rc = view_from_api(ctx, v, in[i], mode);
+        if (rc) return rc;
+        if ((uint32_t)v.e.L > maxL) maxL = (uint32_t)v.e.L;
+        order[i] = {-(double)v.n_rays * (double)(v.prm.settle + 2u * v.prm.accum), i};
+        views[i] = v;
+    }
+    std::stable_sort(order.begin(), order.end());
+    std::vector<ViewDesc> sorted(n);
+    for (unsigned i = 0; i < n; ++i) sorted[i] = views[order[i].second];
+    views.swap(sorted);
+    W = maxL <= 16 ? 16u : 32u;
+    return LYAP_OK;
+}
+
+static int render_views(lyap_ctx *ctx, std::vector<ViewDesc> &views, uint32_t W,
+                        const LyapRenderMode &mode, const LyapTuning &tune)
+{
+    uint64_t rays = 0;
+    for (auto &v : views) rays += v.n_rays;
+    const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
+    LyapTuning t = tune;
+    if (!t.lanes_per_ray && mode.lanes_per_ray) t.lanes_per_ray = mode.lanes_per_ray;
+    const LaunchPlan plan = plan_launch(ctx, rays, t, W, mode.precision, margin);
+    RenderResources &res = ctx_resources(ctx);
+    uint32_t total_tiles = 0;
+    int rc = prepare_views(ctx, res, views.data(), (unsigned)views.size(), plan, W, true, &total_tiles);
+    if (rc) return rc;
+    return enqueue_views(ctx, res, views.data(), (unsigned)views.size(), total_tiles, plan, W, mode.precision);
+}
+
+static LyapTuning merged_tuning(const lyap_ctx *ctx, const LyapTuning *t)
+{
+    LyapTuning m = ctx->tuned;
+    if (t) {
+        if (t->tile_rays) m.tile_rays = t->tile_rays;
+        if (t->waves_per_cu) m.waves_per_cu = t->waves_per_cu;
+        if (t->lanes_per_ray) m.lanes_per_ray = t->lanes_per_ray;
+    }
+    return m;
 }
 
 int launch_render(lyap_ctx *ctx, LyapRGBA *rgba, LyapPoint *points,
@@ -1163,19 +1304,24 @@ int launch_render(lyap_ctx *ctx, LyapRGBA *rgba, LyapPoint *points,
                   unsigned h, float *depth, float *lambda, uint32_t *evals,
                   const LyapRenderMode &mode)
 {
-    const uint64_t rays = 64ull * ((w + 7) / 8) * ((h + 7) / 8);
+    std::vector<ViewDesc> views(1);
+    fill_view(views[0], e, cam, prm, w, h, rgba, points, depth, lambda, evals, lights,
+              num_lights, mode);
     const uint32_t W = e.L <= 16 ? 16u : 32u;
-    const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
-    const LaunchPlan plan = plan_launch(ctx, rays, mode.lanes_per_ray, W, mode.precision, margin);
-    ViewDesc v;
-    fill_view(v, e, cam, prm, w, h, rgba, points, depth, lambda, evals, lights,
-              num_lights, mode, plan.tile_rays, 0);
-    return launch_views(ctx, &v, 1, v.n_tiles, plan, W, mode.precision);
+    return render_views(ctx, views, W, mode, merged_tuning(ctx, nullptr));
 }
 
 } /* namespace lyap */
 
 using namespace lyap;
+
+/* A captured frame: its own resources and the instantiated graph. */
+struct lyap_frame {
+    lyap_ctx *ctx = nullptr;
+    RenderResources res;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
 
 /* Diagnostic: device buffer of 4 x uint64 per wave that the next render
  * launches fill with {rounds, rays started, start clock, end clock}; NULL
@@ -1201,19 +1347,146 @@ extern "C" int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_po
     if (num_lights > LYAP_MAX_LIGHTS) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "more than %d lights", LYAP_MAX_LIGHTS);
     if (num_lights > 0 && !d_lights) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null lights");
     if ((uint64_t)w * h > (1ull << 31)) return fail(ctx, LYAP_ERR_UNSUPPORTED, "image larger than 2^31 pixels");
-    LyapRenderMode mode{};
-    mode.refine = LYAP_REFINE_WALK;
-    mode.precision = LYAP_PREC_PLAIN;
-    if (mode_in) mode = *mode_in;
-    if (mode.refine != LYAP_REFINE_WALK && mode.refine != LYAP_REFINE_BISECT)
-        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown refine mode %d", mode.refine);
-    if (mode.precision < LYAP_PREC_PLAIN || mode.precision > LYAP_PREC_FP64)
-        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown precision %d", mode.precision);
-    if (mode.bisect_steps < 0 || mode.bisect_steps > 64)
-        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "bisect_steps outside 0..64");
+    LyapRenderMode mode;
+    int rc = check_mode(ctx, mode_in, mode);
+    if (rc) return rc;
     EvalDesc e;
     if (!make_eval_desc(&e, *prm, seq, 16))
         return fail(ctx, LYAP_ERR_BAD_SEQUENCE, "sequence must hold 1..%d letter indices 0..3 before the -1", LYAP_MAX_SEQ);
     return launch_render(ctx, d_rgba, d_points, *cam, *prm, e, d_lights, num_lights,
                          w, h, d_depth, d_lambda, d_evals, mode);
+}
+
+extern "C" int lyap_render_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                                 const LyapRenderMode *mode_in, const LyapTuning *tuning)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    if (n_views == 0) return LYAP_OK;
+    if (!views) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null views");
+    LyapRenderMode mode;
+    int rc = check_mode(ctx, mode_in, mode);
+    if (rc) return rc;
+    std::vector<ViewDesc> vd;
+    uint32_t W;
+    rc = build_batch(ctx, views, n_views, mode, vd, W);
+    if (rc) return rc;
+    return render_views(ctx, vd, W, mode, merged_tuning(ctx, tuning));
+}
+
+extern "C" int lyap_autotune(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                             const LyapRenderMode *mode_in, LyapTuning *best)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    if (n_views == 0 || !views) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "no views to tune on");
+    LyapRenderMode mode;
+    int rc = check_mode(ctx, mode_in, mode);
+    if (rc) return rc;
+    std::vector<ViewDesc> vd;
+    uint32_t W;
+    rc = build_batch(ctx, views, n_views, mode, vd, W);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    LYAP_HIP(ctx, hipEventCreate(&e0));
+    LYAP_HIP(ctx, hipEventCreate(&e1));
+    /* candidates: lanes per ray x tile length in generations */
+    const uint32_t lanes[] = {2u, 4u, 8u, 16u};
+    const uint32_t gens[] = {1u, 2u, 4u};
+    float best_ms = 1e30f;
+    LyapTuning best_t{};
+    for (uint32_t k : lanes) {
+        for (uint32_t g : gens) {
+            LyapTuning t{};
+            t.lanes_per_ray = k;
+            t.tile_rays = g * (64u / k);
+            std::vector<ViewDesc> trial = vd;
+            rc = render_views(ctx, trial, W, mode, t);           /* warm-up */
+            if (rc) break;
+            LYAP_HIP(ctx, hipEventRecord(e0, ctx->stream));
+            for (int rep = 0; rep < 2 && rc == LYAP_OK; ++rep) {
+                trial = vd;
+                rc = render_views(ctx, trial, W, mode, t);
+            }
+            if (rc) break;
+            LYAP_HIP(ctx, hipEventRecord(e1, ctx->stream));
+            LYAP_HIP(ctx, hipEventSynchronize(e1));
+            float ms = 0.f;
+            LYAP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) { best_ms = ms; best_t = t; }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    ctx->tuned = best_t;
+    if (best) *best = best_t;
+    return LYAP_OK;
+}
+
+extern "C" int lyap_frame_capture(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                                  const LyapRenderMode *mode_in, const LyapTuning *tuning,
+                                  lyap_frame **out)
+{
+    if (!ctx || !out) return LYAP_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (n_views == 0 || !views) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "no views");
+    LyapRenderMode mode;
+    int rc = check_mode(ctx, mode_in, mode);
+    if (rc) return rc;
+    std::vector<ViewDesc> vd;
+    uint32_t W;
+    rc = build_batch(ctx, views, n_views, mode, vd, W);
+    if (rc) return rc;
+    uint64_t rays = 0;
+    for (auto &v : vd) rays += v.n_rays;
+    const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
+    const LaunchPlan plan = plan_launch(ctx, rays, merged_tuning(ctx, tuning), W, mode.precision, margin);
+    lyap_frame *f = new lyap_frame();
+    f->ctx = ctx;
+    uint32_t total_tiles = 0;
+    rc = prepare_views(ctx, f->res, vd.data(), n_views, plan, W, true, &total_tiles);
+    if (rc == LYAP_OK) {
+        /* everything the frame needs exists now; record without syncing */
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            rc = fail(ctx, LYAP_ERR_DEVICE, "stream capture: %s", hipGetErrorString(e));
+        } else {
+            rc = enqueue_views(ctx, f->res, vd.data(), n_views, total_tiles, plan, W, mode.precision);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc == LYAP_OK && e != hipSuccess)
+                rc = fail(ctx, LYAP_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+            f->graph = g;
+            if (rc == LYAP_OK) {
+                e = hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0);
+                if (e != hipSuccess)
+                    rc = fail(ctx, LYAP_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+            }
+        }
+    }
+    if (rc != LYAP_OK) {
+        lyap_frame_destroy(f);
+        return rc;
+    }
+    *out = f;
+    return LYAP_OK;
+}
+
+extern "C" int lyap_frame_replay(lyap_ctx *ctx, lyap_frame *f)
+{
+    if (!ctx || !f || f->ctx != ctx || !f->exec) return LYAP_ERR_INVALID_ARGUMENT;
+    LYAP_HIP(ctx, hipGraphLaunch(f->exec, ctx->stream));
+    ctx->last_counters = f->res.counters;
+    return LYAP_OK;
+}
+
+extern "C" void lyap_frame_destroy(lyap_frame *f)
+{
+    if (!f) return;
+    if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
+    if (f->exec) (void)hipGraphExecDestroy(f->exec);
+    if (f->graph) (void)hipGraphDestroy(f->graph);
+    if (f->ctx) release_resources(f->ctx, f->res);
+    delete f;
 }

@@ -5,11 +5,6 @@
 #include "lyap.h"
 
 extern "C" {
-int lyap_render_batch(lyap_ctx *, const LyapView *, unsigned, const LyapRenderMode *, const LyapTuning *) { return LYAP_ERR_UNSUPPORTED; }
-int lyap_autotune(lyap_ctx *, const LyapView *, unsigned, const LyapRenderMode *, LyapTuning *) { return LYAP_ERR_UNSUPPORTED; }
-int lyap_frame_capture(lyap_ctx *, const LyapView *, unsigned, const LyapRenderMode *, const LyapTuning *, lyap_frame **out) { if (out) *out = 0; return LYAP_ERR_UNSUPPORTED; }
-int lyap_frame_replay(lyap_ctx *, lyap_frame *) { return LYAP_ERR_UNSUPPORTED; }
-void lyap_frame_destroy(lyap_frame *) {}
 int lyap_server_create(lyap_ctx *, const LyapServerConfig *, lyap_server **out) { if (out) *out = 0; return LYAP_ERR_UNSUPPORTED; }
 void lyap_server_destroy(lyap_server *) {}
 int lyap_server_submit(lyap_server *, const LyapRequest *, uint64_t *) { return LYAP_ERR_UNSUPPORTED; }

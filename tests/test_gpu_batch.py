@@ -1,0 +1,139 @@
+"""Batched views, the autotuner and captured frames on the GPU.
+
+ - a batch of views with different sizes, sequences (periods 3..12 and one
+   of 20), iteration counts and cameras gives, view by view, exactly the
+   bytes of rendering each view alone (scheduling never changes a ray's
+   arithmetic);
+ - FP64 batches reproduce the oracle exactly for sampled views;
+ - a captured frame replayed several times, with other work and
+   synchronisations between replays, matches the eager batch every time.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import lyapunov3d_amd as lyap
+from oracle import oracle as O
+from tests import workloads as W
+from tests.gpu_util import RenderBuffers, device_lights, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    torch_cuda()
+    c = lyap.Context(0)
+    yield c
+    c.close()
+
+
+def small_batch(n=10, seed=7, sides=(24, 40, 64, 96, 17), accum=(60, 200)):
+    views, lights, n_lights = W.batch_views(n, seed=seed, sides=sides, accum_range=accum, settle=40)
+    if n > 3:
+        views[3]["seq"] = "ABCABCABCABCABCABCAB"   # period 20: the 32-wide window
+    return views, lights, n_lights
+
+
+def alloc(views):
+    return [RenderBuffers(v["w"], v["h"]) for v in views]
+
+
+def as_api(views, bufs, ld, n_lights):
+    out = []
+    for v, b in zip(views, bufs):
+        out.append(lyap.make_view(v["cam"], v["prm"], v["seq"], v["w"], v["h"], b.rgba.data_ptr(),
+                                  b.points.data_ptr(), b.depth.data_ptr(), b.lam.data_ptr(),
+                                  b.evals.data_ptr(), ld.data_ptr(), n_lights))
+    return out
+
+
+def same(a, b):
+    for k in ("rgba", "depth", "lambda", "evals", "points"):
+        assert np.array_equal(a[k], b[k], equal_nan=True), k
+
+
+@pytest.mark.parametrize("prec", [lyap.PREC_PRODUCT, lyap.PREC_FP64])
+def test_batch_equals_single_views(ctx, prec):
+    views, lights, n_lights = small_batch()
+    ld = device_lights(lights, n_lights)
+    mode = W.c2_mode(precision=prec, fp64_margin=0.0)
+    bufs = alloc(views)
+    ctx.render_batch(as_api(views, bufs, ld, n_lights), mode)
+    ctx.sync()
+    batch_evals = ctx.last_evals()
+    got = [b.fetch() for b in bufs]
+    total = 0
+    for v, g in zip(views, got):
+        one = RenderBuffers(v["w"], v["h"])
+        one.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, mode)
+        ctx.sync()
+        same(g, one.fetch())
+        total += ctx.last_evals()
+    assert batch_evals == total
+    if prec == lyap.PREC_FP64:
+        for i in (0, 3, 5):
+            v = views[i]
+            ref = O.render(v["cam"], v["prm"], v["seq"], lights, n_lights, v["w"], v["h"], mode=mode, lam=O.F64)
+            assert np.array_equal(got[i]["evals"], ref["evals"])
+            assert np.array_equal(got[i]["depth"], ref["depth"], equal_nan=True)
+
+
+def test_frame_replays_match_eager(ctx):
+    torch = torch_cuda()
+    views, lights, n_lights = small_batch(seed=11)
+    ld = device_lights(lights, n_lights)
+    mode = W.c2_mode(fp64_margin=0.0)
+    eager_bufs = alloc(views)
+    ctx.render_batch(as_api(views, eager_bufs, ld, n_lights), mode)
+    ctx.sync()
+    eager = [b.fetch() for b in eager_bufs]
+    bufs = alloc(views)
+    frame = ctx.frame_capture(as_api(views, bufs, ld, n_lights), mode)
+    for rep in range(4):
+        for b in bufs:                       # scrub the outputs between replays
+            b.rgba.zero_(); b.depth.fill_(7.0); b.evals.zero_()
+        torch.cuda.synchronize()
+        frame.replay()
+        ctx.sync()
+        # unrelated eager work on the same context between replays
+        other = RenderBuffers(16, 16)
+        other.render(ctx, views[0]["cam"], views[0]["prm"], "AB", ld, n_lights, mode)
+        ctx.sync()
+        for b, e in zip(bufs, eager):
+            same(b.fetch(), e)
+    frame.close()
+
+
+def test_autotune_keeps_results(ctx):
+    views, lights, n_lights = small_batch(n=6, seed=3)
+    ld = device_lights(lights, n_lights)
+    mode = W.c2_mode(fp64_margin=0.0)
+    bufs = alloc(views)
+    api = as_api(views, bufs, ld, n_lights)
+    ctx.render_batch(api, mode)
+    ctx.sync()
+    before = [b.fetch() for b in bufs]
+    best = ctx.autotune(api, mode)
+    assert best.lanes_per_ray in (2, 4, 8, 16) and best.tile_rays > 0
+    ctx.render_batch(api, mode)
+    ctx.sync()
+    for b, e in zip(bufs, before):
+        same(b.fetch(), e)
+
+
+def test_batch_errors(ctx):
+    views, lights, n_lights = small_batch(n=2)
+    ld = device_lights(lights, n_lights)
+    bufs = alloc(views)
+    api = as_api(views, bufs, ld, n_lights)
+    api[1].seq[0] = 9
+    with pytest.raises(lyap.LyapError) as e:
+        ctx.render_batch(api)
+    assert e.value.status == lyap.ERR_BAD_SEQUENCE
+    api = as_api(views, bufs, ld, n_lights)
+    api[0].rgba = None
+    with pytest.raises(lyap.LyapError) as e:
+        ctx.render_batch(api)
+    assert e.value.status == lyap.ERR_INVALID_ARGUMENT
