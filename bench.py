@@ -19,7 +19,13 @@ Also reported (same JSON line, "workloads"):
   c4  64 views of seeded size 128..1024 in one batched launch sequence
       (camera radius 3..6, sequences of 3-12 letters, 200..2000 iterations),
       against rendering the same views one request at a time in the plain
-      FP32 per-step-log form.
+      FP32 per-step-log form;
+  c5  the serving stream: Poisson arrivals at 200 requests/s (70% views of
+      side 256/512/1024, 30% volume blocks of edge 32..256 inside [2,4]^3,
+      200..4000 iterations) paced through the server for --c5-seconds
+      (p50/p99 latency, served vs offered rays/s), then the first
+      --c5-capacity requests offered at once to the server and to plain
+      per-request calls.
 The roofline uses algorithmic work: 3*settle + 6*accum FLOP per lambda
 evaluation, against MI355X's FP32 vector peak of 157.3 TFLOP/s. The CPU
 baseline is the oracle's FP32-as-written render of a bounded sample of rows,
@@ -55,8 +61,12 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="approximate CPU budget of the baseline sample")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--workloads", default="c2,c1,c3,c4", help="extra workloads ('' for none)")
+    p.add_argument("--workloads", default="c2,c1,c3,c4,c5", help="extra workloads ('' for none)")
     p.add_argument("--c4-views", type=int, default=64)
+    p.add_argument("--c5-seconds", type=float, default=3.0,
+                   help="length of the paced request stream (the full workload is 60)")
+    p.add_argument("--c5-capacity", type=int, default=120,
+                   help="requests offered at once for the served vs plain capacity")
     return p.parse_args()
 
 
@@ -187,19 +197,173 @@ def bench_batch(lyap, timer, ctx, n_views):
     frame = ctx.frame_capture(api, mode)
     graph_ms = timer.run(lambda: frame.replay(), 3)
     frame.close()
-    # the same views one request at a time, plain FP32 per-step log
-    plain_mode = W.c2_mode(precision=lyap.PREC_PLAIN, fp64_margin=0.0)
-
-    def plain_all():
+    # the same views one request at a time, plain FP32 per-step log, in the
+    # reference's launch shape and in the batched schedule
+    def plain_all(schedule):
+        m = W.c2_mode(precision=lyap.PREC_PLAIN, fp64_margin=0.0)
+        m.schedule = schedule
         for v, b in zip(views, bufs):
-            b.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, plain_mode)
-    plain_ms = timer.run(plain_all, 1)
+            b.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, m)
+    plain_ms = timer.run(lambda: plain_all(lyap.SCHEDULE_DIRECT), 1)
+    plain_batched_ms = timer.run(lambda: plain_all(lyap.SCHEDULE_BATCHED), 1)
     log("c4 done")
     return {"views": n_views, "rays": rays, "ms": ms, "graph_replay_ms": graph_ms,
             "rays_per_s": rays / (ms * 1e-3), "lambda_evals_per_s": evals / (ms * 1e-3),
             "tflops": flop / (ms * 1e-3) / 1e12, "roofline_frac": flop / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
             "plain_per_request_ms": plain_ms, "plain_per_request_rays_per_s": rays / (plain_ms * 1e-3),
-            "speedup_vs_plain_per_request": plain_ms / ms}
+            "speedup_vs_plain_per_request": plain_ms / ms,
+            "plain_batched_schedule_ms": plain_batched_ms,
+            "speedup_vs_plain_batched_schedule": plain_batched_ms / ms}
+
+
+def _pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q)) if len(xs) else None
+
+
+def _make_requests(lyap, reqs, ld, n_lights, view_deadline_ms, volume_deadline_ms):
+    out = []
+    for q in reqs:
+        if q["kind"] == "view":
+            v = lyap.make_view(q["cam"], q["prm"], q["seq"], q["w"], q["h"], 0, 0, 0, 0, 0,
+                               ld.data_ptr(), n_lights)
+            out.append(lyap.view_request(v, deadline_ms=view_deadline_ms))
+        else:
+            out.append(lyap.volume_request(q["prm"], q["seq"], q["lo"], q["hi"], q["n"],
+                                           deadline_ms=volume_deadline_ms))
+    return out
+
+
+def bench_serving(lyap, timer, ctx, seconds, sat_n):
+    """The serving workload: a paced Poisson stream through the server
+    (latency), then the first `sat_n` requests offered at once to the server
+    and to plain per-request launches (capacity)."""
+    import torch
+    from tests import workloads as W
+    from tests.gpu_util import RenderBuffers, device_lights, empty
+    reqs, lights, n_lights = W.serving_stream(seconds=seconds)
+    ld = device_lights(lights, n_lights)
+    mode = W.c2_mode(fp64_margin=0.0)
+    vmode = lyap.volume_mode(**W.C3_MODE)
+    cfg = lyap.server_config(hbm_budget_bytes=64 << 30, mode=mode, vol_mode=vmode)
+    api = _make_requests(lyap, reqs, ld, n_lights, 50.0, 250.0)
+    torch.cuda.synchronize()
+    srv = lyap.Server(ctx, cfg)
+    # warm-up: calibrates the server's cost rates
+    for r in api[:12]:
+        srv.submit(r)
+    srv.drain(60000.0)
+    for t in srv.finished():
+        srv.release(t)
+    st0 = srv.stats()
+    log(f"c5 paced: {len(api)} requests over {seconds:.1f} s")
+
+    # ---- paced stream
+    t0 = srv.clock_ms()
+    arrival, kind_of = {}, {}
+    lat = {"view": [], "volume": [], "view_1024": [], "view_512": [], "view_256": []}
+    i, last_finish, n_done = 0, t0, 0
+    while n_done < len(api):
+        now = srv.clock_ms() - t0
+        while i < len(api) and reqs[i]["t_ms"] <= now:
+            t = srv.submit(api[i])
+            arrival[t] = t0 + reqs[i]["t_ms"]
+            kind_of[t] = i
+            i += 1
+        srv.progress(0.2)
+        for t in srv.finished():
+            r = srv.poll(t)
+            if r.status != lyap.OK:
+                raise RuntimeError(f"request {kind_of[t]} ended with status {r.status}")
+            q = reqs[kind_of[t]]
+            ms = r.finish_ms - arrival[t]
+            lat[q["kind"]].append(ms)
+            if q["kind"] == "view":
+                lat[f"view_{q['w']}"].append(ms)
+            last_finish = max(last_finish, r.finish_ms)
+            srv.release(t)
+            n_done += 1
+    st1 = srv.stats()
+    wall_s = (last_finish - t0) * 1e-3
+    view_rays = sum(q["w"] * q["h"] for q in reqs if q["kind"] == "view")
+    voxels = sum(q["n"] ** 3 for q in reqs if q["kind"] == "volume")
+    paced = {
+        "requests": len(api), "stream_seconds": seconds, "wall_seconds": wall_s,
+        "offered_rays_per_s": view_rays / seconds, "offered_voxels_per_s": voxels / seconds,
+        "served_rays_per_s": view_rays / wall_s, "served_voxels_per_s": voxels / wall_s,
+        "lambda_evals_per_s": (st1.evals - st0.evals) / wall_s,
+        "device_busy_ms": st1.device_ms - st0.device_ms,
+        "launches": st1.launches - st0.launches,
+        "peak_resident_bytes": st1.peak_resident_bytes,
+        "latency_ms": {k: {"n": len(v), "p50": _pct(v, 50), "p99": _pct(v, 99), "max": max(v) if v else None}
+                       for k, v in lat.items()},
+    }
+    log(f"c5 paced done in {wall_s:.2f} s")
+
+    # ---- capacity: the first sat_n requests offered at once
+    sub = api[:sat_n]
+    sub_q = reqs[:sat_n]
+    sub_rays = sum(q["w"] * q["h"] for q in sub_q if q["kind"] == "view")
+    sub_vox = sum(q["n"] ** 3 for q in sub_q if q["kind"] == "volume")
+    torch.cuda.synchronize()
+    e0 = srv.stats().evals
+    t = time.perf_counter()
+    for r in sub:
+        srv.submit(r)
+    srv.drain(600000.0)
+    served_s = time.perf_counter() - t
+    served_evals = srv.stats().evals - e0
+    for tk in srv.finished():
+        srv.release(tk)
+    srv.close()
+    log(f"c5 served capacity: {served_s:.2f} s")
+
+    # plain per-request: each request its own call, one after another, in
+    # the plain FP32 per-step-log form; views in the reference's launch shape
+    # (one lane per ray) and, for comparison, in the batched schedule
+    vol_out = empty(4 * 256 ** 3)
+    bufs = {}
+
+    def plain(schedule):
+        m = W.c2_mode(precision=lyap.PREC_PLAIN, fp64_margin=0.0)
+        m.schedule = schedule
+        vm = lyap.volume_mode(lyap.PREC_PLAIN)
+        evals = 0
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for q in sub_q:
+            if q["kind"] == "view":
+                b = bufs.get(q["w"])
+                if b is None:
+                    b = bufs[q["w"]] = RenderBuffers(q["w"], q["h"])
+                b.render(ctx, q["cam"], q["prm"], q["seq"], ld, n_lights, m)
+                ctx.sync()
+                evals += ctx.last_evals()
+            else:
+                ctx.calc_volume(vol_out.data_ptr(), q["prm"], q["seq"], q["lo"], q["hi"], q["n"], vm)
+                ctx.sync()
+                evals += q["n"] ** 3
+        return time.perf_counter() - t, evals
+
+    direct_s, direct_evals = plain(lyap.SCHEDULE_DIRECT)
+    log(f"c5 plain per-request (direct): {direct_s:.2f} s")
+    batched_s, batched_evals = plain(lyap.SCHEDULE_BATCHED)
+    log(f"c5 plain per-request (batched schedule): {batched_s:.2f} s")
+    capacity = {
+        "requests": len(sub), "view_rays": sub_rays, "voxels": sub_vox,
+        "served_s": served_s, "served_rays_per_s": sub_rays / served_s,
+        "served_lambda_evals_per_s": served_evals / served_s,
+        "plain_per_request_s": direct_s, "plain_per_request_rays_per_s": sub_rays / direct_s,
+        "plain_per_request_lambda_evals_per_s": direct_evals / direct_s,
+        "served_over_plain_rays_per_s": direct_s / served_s,
+        "plain_batched_schedule_s": batched_s,
+        "plain_batched_schedule_rays_per_s": sub_rays / batched_s,
+        "served_over_plain_batched_schedule": batched_s / served_s,
+    }
+    return {"paced": paced, "capacity": capacity,
+            "config": "views: product form, 8 bisections, no FP64 margin; volumes: product form with "
+                      "FP64 re-evaluation where |lambda| < 0.01; deadlines 50 ms (views) / 250 ms "
+                      "(volumes); 64 GiB budget; plain: per-step-log FP32, one synchronous call per request, "
+                      "views in the reference's launch shape (one lane per ray, 8x8 blocks)"}
 
 
 def main():
@@ -297,6 +461,8 @@ def main():
         extra.update(bench_volumes(lyap, timer, ctx, which))
         if "c4" in which:
             extra["c4_batched"] = bench_batch(lyap, timer, ctx, args.c4_views)
+        if "c5" in which:
+            extra["c5_serving"] = bench_serving(lyap, timer, ctx, args.c5_seconds, args.c5_capacity)
         line["workloads"] = extra
     if not args.no_cpu:
         log("cpu baseline")

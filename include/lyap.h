@@ -189,8 +189,17 @@ typedef struct LyapRenderMode {
     float fp64_margin;      /* BISECT: samples with |lambda - opaque| < margin
                                are re-evaluated in FP64; 0 = off */
     uint32_t lanes_per_ray; /* 0 = chosen by the library */
-    uint32_t reserved[2];
+    int32_t schedule;       /* LYAP_SCHEDULE_* */
+    uint32_t reserved;
 } LyapRenderMode;
+
+/* How rays are spread over lanes. BATCHED: the persistent work queue with
+ * lane groups per ray, refill of finished lanes and continuation launches.
+ * DIRECT: the reference's own launch shape (kernel.cu:500-516): one lane
+ * per ray for 64 consecutive rays (an 8x8 pixel block), each lane marching
+ * its ray to the end and a wave taking new rays only when all 64 are done.
+ * Both give the same bytes; DIRECT is the plain per-request baseline. */
+enum { LYAP_SCHEDULE_BATCHED = 0, LYAP_SCHEDULE_DIRECT = 1 };
 
 /* kernel_calc_render (kernel.hpp:17, kernel.cu:500-516; launched at
  * lyap_interactive.cu:711): one primary ray per pixel of a w*h image through
@@ -286,9 +295,14 @@ typedef struct LyapRequest {
 } LyapRequest;
 
 typedef struct LyapServerConfig {
-    size_t hbm_budget_bytes;           /* admission cap for resident state */
-    uint32_t max_inflight;             /* launches queued on the device */
-    uint32_t slice_tiles;              /* tiles per launch (0: derived) */
+    size_t hbm_budget_bytes;           /* cap on resident outputs (0: none) */
+    uint32_t render_slots;             /* render launches in flight, one
+                                          stream each (0: 2) */
+    uint32_t volume_slots;             /* volume launches in flight (0: 1) */
+    double slice_ms;                   /* device time aimed at per launch (0: 12) */
+    uint32_t chunk_rays;               /* smallest piece of a view a launch
+                                          takes, a multiple of 64 (0: 4096) */
+    uint32_t reserved;
     LyapRenderMode mode;
     LyapVolumeMode vol_mode;
 } LyapServerConfig;
@@ -297,25 +311,56 @@ typedef struct LyapResult {
     int32_t status;                    /* LYAP_OK, LYAP_ERR_PENDING, ... */
     uint32_t kind;
     double submit_ms, start_ms, finish_ms;   /* server clock */
-    uint64_t rays, voxels, evals;
-    void *rgba;                        /* resident outputs (device) */
-    float *lambda;
+    uint64_t rays, voxels;
+    uint64_t evals;                    /* volumes: lambda evaluations; views: 0
+                                          (a launch mixes views, so evaluations
+                                          are counted per server, not per view) */
+    void *rgba;                        /* view RGBA (device) */
+    float *lambda;                     /* view lambda, or the volume (device) */
 } LyapResult;
 
+typedef struct LyapServerStats {
+    uint64_t submitted, completed, cancelled, failed;
+    uint64_t rays, voxels, evals;      /* work of finished launches */
+    uint64_t launches;
+    size_t resident_bytes, peak_resident_bytes;
+    uint32_t queued, active;           /* requests waiting / admitted */
+    double device_ms;                  /* sum of launch durations */
+} LyapServerStats;
+
+/* Continuous batching over one device. A view is cut into 64-ray pieces
+ * and a volume into z slabs; each launch takes the pieces of the requests
+ * with the earliest deadlines, up to about `slice_ms` of device time, so a
+ * request that arrives while others run joins the next launch. Render
+ * launches go to their own streams and volume launches to another, so their
+ * work overlaps on the device. A request's outputs are admitted against
+ * `hbm_budget_bytes` when its first piece is issued and stay resident until
+ * lyap_server_release. One host thread drives a server. */
 typedef struct lyap_server lyap_server;
 int lyap_server_create(lyap_ctx *ctx, const LyapServerConfig *cfg,
                        lyap_server **out);
 void lyap_server_destroy(lyap_server *srv);
-/* Queues a request; admission against the budget happens when it starts. */
+/* Queues a request. A VIEW's lights and any caller-given outputs must stay
+ * valid until the request finishes. */
 int lyap_server_submit(lyap_server *srv, const LyapRequest *req,
                        uint64_t *ticket);
 /* Advances the server: retires finished launches, admits requests, and
- * issues new launches. Never blocks longer than `wait_ms`. */
+ * issues new launches. Returns once something changed, or after `wait_ms`. */
 int lyap_server_progress(lyap_server *srv, double wait_ms);
+/* LYAP_OK with out->status LYAP_ERR_PENDING while the request runs. */
 int lyap_server_poll(lyap_server *srv, uint64_t ticket, LyapResult *out);
+/* Frees a finished request's resident outputs and forgets its ticket. */
 int lyap_server_release(lyap_server *srv, uint64_t ticket);
+/* Stops issuing a request's pieces; it ends with LYAP_ERR_CANCELLED once
+ * the pieces already on the device are done. */
 int lyap_server_cancel(lyap_server *srv, uint64_t ticket);
 double lyap_server_clock_ms(lyap_server *srv);
+int lyap_server_stats(lyap_server *srv, LyapServerStats *out);
+/* Tickets of requests that finished (any status) since the last call, in
+ * the order they finished: up to `cap` of them, their number in *n. */
+int lyap_server_finished(lyap_server *srv, uint64_t *tickets, unsigned cap, unsigned *n);
+/* Blocks until every request has finished (or `timeout_ms` passes). */
+int lyap_server_drain(lyap_server *srv, double timeout_ms);
 
 #ifdef __cplusplus
 } /* extern "C" */

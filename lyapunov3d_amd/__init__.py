@@ -48,6 +48,9 @@ REFINE_BISECT = 1
 REQ_VIEW = 0
 REQ_VOLUME = 1
 
+SCHEDULE_BATCHED = 0
+SCHEDULE_DIRECT = 1
+
 
 # ---------------------------------------------------------------- types ----
 
@@ -120,7 +123,7 @@ class RenderMode(C.Structure):
     _fields_ = [
         ("refine", C.c_int32), ("bisect_steps", C.c_int32), ("normal_h", C.c_float),
         ("precision", C.c_int32), ("fp64_margin", C.c_float),
-        ("lanes_per_ray", C.c_uint32), ("reserved", C.c_uint32 * 2),
+        ("lanes_per_ray", C.c_uint32), ("schedule", C.c_int32), ("reserved", C.c_uint32),
     ]
 
 
@@ -152,8 +155,9 @@ class Request(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [
-        ("hbm_budget_bytes", C.c_size_t), ("max_inflight", C.c_uint32),
-        ("slice_tiles", C.c_uint32), ("mode", RenderMode), ("vol_mode", VolumeMode),
+        ("hbm_budget_bytes", C.c_size_t), ("render_slots", C.c_uint32),
+        ("volume_slots", C.c_uint32), ("slice_ms", C.c_double), ("chunk_rays", C.c_uint32),
+        ("reserved", C.c_uint32), ("mode", RenderMode), ("vol_mode", VolumeMode),
     ]
 
 
@@ -163,6 +167,16 @@ class Result(C.Structure):
         ("start_ms", C.c_double), ("finish_ms", C.c_double), ("rays", C.c_uint64),
         ("voxels", C.c_uint64), ("evals", C.c_uint64), ("rgba", C.c_void_p),
         ("lambda_", C.c_void_p),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("completed", C.c_uint64), ("cancelled", C.c_uint64),
+        ("failed", C.c_uint64), ("rays", C.c_uint64), ("voxels", C.c_uint64),
+        ("evals", C.c_uint64), ("launches", C.c_uint64), ("resident_bytes", C.c_size_t),
+        ("peak_resident_bytes", C.c_size_t), ("queued", C.c_uint32), ("active", C.c_uint32),
+        ("device_ms", C.c_double),
     ]
 
 
@@ -225,6 +239,9 @@ _SIGS = {
     "lyap_server_release": (C.c_int, [_P, C.c_uint64]),
     "lyap_server_cancel": (C.c_int, [_P, C.c_uint64]),
     "lyap_server_clock_ms": (C.c_double, [_P]),
+    "lyap_server_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "lyap_server_drain": (C.c_int, [_P, C.c_double]),
+    "lyap_server_finished": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_uint, C.POINTER(C.c_uint)]),
 }
 
 
@@ -346,8 +363,9 @@ def volume_mode(precision=PREC_PLAIN, renorm=16, fp64_margin=0.0, converge_tau=0
 
 
 def render_mode(refine=REFINE_WALK, bisect_steps=8, normal_h=0.0, precision=PREC_PLAIN,
-                fp64_margin=0.0, lanes_per_ray=0) -> RenderMode:
+                fp64_margin=0.0, lanes_per_ray=0, schedule=SCHEDULE_BATCHED) -> RenderMode:
     m = RenderMode()
+    m.schedule = schedule
     m.refine = refine
     m.bisect_steps = bisect_steps
     m.normal_h = normal_h
@@ -501,3 +519,108 @@ def make_view(cam: LyapCam, prm: LyapParams, seq, w: int, h: int, rgba: int, poi
     v.rgba, v.points, v.depth, v.lambda_, v.evals = rgba, points or None, depth or None, lam or None, evals or None
     v.lights, v.num_lights = lights or None, num_lights
     return v
+
+
+# ------------------------------------------------------------- serving ----
+
+def view_request(view: View, deadline_ms: float = 0.0) -> Request:
+    """A VIEW request; leave view.rgba 0 to have the server keep RGBA and lambda resident."""
+    r = Request()
+    r.kind = REQ_VIEW
+    r.view = view
+    r.deadline_ms = deadline_ms
+    return r
+
+
+def volume_request(prm: LyapParams, seq, lo, hi, n, out: int = 0, deadline_ms: float = 0.0) -> Request:
+    """A VOLUME request over the box [lo, hi) with n (or (nx, ny, nz)) samples per axis."""
+    r = Request()
+    r.kind = REQ_VOLUME
+    r.vol_prm = prm
+    s = list(seq_array(seq))
+    for i, x in enumerate(s):
+        r.vol_seq[i] = x
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+    for k in range(3):
+        r.vol_lo[k] = lo[k]
+        r.vol_hi[k] = hi[k]
+    r.vol_n[0], r.vol_n[1], r.vol_n[2] = nx, ny, nz
+    r.vol_out = out or None
+    r.deadline_ms = deadline_ms
+    return r
+
+
+def server_config(hbm_budget_bytes: int = 0, render_slots: int = 0, volume_slots: int = 0,
+                  slice_ms: float = 0.0, chunk_rays: int = 0, mode: Optional[RenderMode] = None,
+                  vol_mode: Optional[VolumeMode] = None) -> ServerConfig:
+    c = ServerConfig()
+    c.hbm_budget_bytes = hbm_budget_bytes
+    c.render_slots, c.volume_slots = render_slots, volume_slots
+    c.slice_ms, c.chunk_rays = slice_ms, chunk_rays
+    c.mode = mode if mode is not None else render_mode(REFINE_BISECT, 8, 1e-3, PREC_PRODUCT)
+    c.vol_mode = vol_mode if vol_mode is not None else volume_mode(PREC_PRODUCT)
+    return c
+
+
+class Server:
+    """lyap_server: continuous batching of view and volume requests on the context's device."""
+
+    def __init__(self, ctx: Context, cfg: Optional[ServerConfig] = None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        _check(self.lib.lyap_server_create(ctx.h, C.byref(cfg) if cfg is not None else None, C.byref(h)),
+               ctx.h, "lyap_server_create")
+        self.h = h
+
+    def submit(self, req: Request) -> int:
+        t = C.c_uint64()
+        _check(self.lib.lyap_server_submit(self.h, C.byref(req), C.byref(t)), self.ctx.h, "lyap_server_submit")
+        return t.value
+
+    def progress(self, wait_ms: float = 0.0):
+        _check(self.lib.lyap_server_progress(self.h, wait_ms), self.ctx.h, "lyap_server_progress")
+
+    def poll(self, ticket: int) -> Result:
+        r = Result()
+        _check(self.lib.lyap_server_poll(self.h, ticket, C.byref(r)), self.ctx.h, "lyap_server_poll")
+        return r
+
+    def release(self, ticket: int):
+        _check(self.lib.lyap_server_release(self.h, ticket), self.ctx.h, "lyap_server_release")
+
+    def cancel(self, ticket: int):
+        _check(self.lib.lyap_server_cancel(self.h, ticket), self.ctx.h, "lyap_server_cancel")
+
+    def clock_ms(self) -> float:
+        return float(self.lib.lyap_server_clock_ms(self.h))
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        _check(self.lib.lyap_server_stats(self.h, C.byref(s)), self.ctx.h, "lyap_server_stats")
+        return s
+
+    def finished(self, cap: int = 256) -> list:
+        """Tickets finished since the last call, in finishing order."""
+        out = []
+        buf = (C.c_uint64 * cap)()
+        n = C.c_uint()
+        while True:
+            _check(self.lib.lyap_server_finished(self.h, buf, cap, C.byref(n)), self.ctx.h, "lyap_server_finished")
+            out.extend(buf[i] for i in range(n.value))
+            if n.value < cap:
+                return out
+
+    def drain(self, timeout_ms: float = 0.0):
+        _check(self.lib.lyap_server_drain(self.h, timeout_ms), self.ctx.h, "lyap_server_drain")
+
+    def close(self):
+        if self.h:
+            self.lib.lyap_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

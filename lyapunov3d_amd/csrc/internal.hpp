@@ -55,6 +55,20 @@ int launch_volume(lyap_ctx *ctx, float *out, const EvalDesc &e,
 
 void free_render_resources(lyap_ctx *ctx);
 
+int launch_volume_slab(lyap_ctx *ctx, float *out, const EvalDesc &e,
+                       const float lo[3], const float hi[3], unsigned nx,
+                       unsigned ny, unsigned nz, unsigned z_begin, unsigned z_end,
+                       const float *xyz, size_t npoints, const LyapVolumeMode &mode);
+
+/* Server launch slots (render.hip). */
+struct RenderSlot;
+RenderSlot *render_slot_create();
+void render_slot_destroy(lyap_ctx *ctx, RenderSlot *slot);
+const uint32_t *render_slot_counters(const RenderSlot *slot);
+int render_slot_launch(lyap_ctx *ctx, RenderSlot *slot, const LyapView *views,
+                       const uint32_t *ray_begin, const uint32_t *ray_count, unsigned n,
+                       const LyapRenderMode &mode, const LyapTuning &tune);
+
 int launch_render(lyap_ctx *ctx, LyapRGBA *rgba, LyapPoint *points,
                   const LyapCam &cam, const LyapParams &prm, const EvalDesc &e,
                   const LyapLight *lights, unsigned num_lights, unsigned w,

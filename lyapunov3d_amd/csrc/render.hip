@@ -52,6 +52,7 @@ struct ViewDesc {
     float normal_h;
     float margin;              /* FP64 re-evaluation margin during bisection */
     uint32_t max_samples;
+    uint32_t ray_begin;        /* first ray of this launch (a view may be split) */
     uint32_t cont_base;        /* this view's region of the continuation buffers */
     uint32_t cont_cap;
 };
@@ -83,7 +84,7 @@ struct RenderArgs {
     uint32_t level;
     uint32_t budget;
     uint32_t resident_lanes;
-    uint32_t pad2;
+    uint32_t direct;           /* one lane per ray, no refill until the wave is idle */
     const ContEntry *cont_in;
     const uint32_t *in_counts;         /* per view */
     ContEntry *cont_out;
@@ -324,6 +325,7 @@ __device__ void write_hit(const ViewDesc &v, const Ray &r, F3 N)
  * for a padding ray past the image edge) when no evaluation is needed. */
 __device__ bool start_ray(const ViewDesc &v, uint32_t id, Ray &r)
 {
+    id += v.ray_begin;
     const uint32_t blk = id >> 6, in = id & 63u;
     const uint32_t px = (blk % v.blocks_x) * 8u + (in & 7u);
     const uint32_t py = (blk / v.blocks_x) * 8u + (in >> 3);
@@ -808,6 +810,8 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
         /* ---- refill idle groups (one ray per group of K lanes) ---------- */
         bool started = false;
         for (;;) {
+            /* the reference's shape: a wave's rays all end before new ones start */
+            if (a.direct && __ballot(r.phase != PH_IDLE) != 0) break;
             const uint32_t rank = lane % K;
             const bool need = rank == 0 && lane + K <= 64u && r.phase == PH_IDLE;
             const uint64_t mask = __ballot(need);
@@ -863,7 +867,7 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
             continue;
         }
         /* ---- no more rays for this view: spread idle lanes over busy rays */
-        const bool starved = pos >= end && (queue_empty || pend_tile != 0xFFFFFFFFu);
+        const bool starved = !a.direct && pos >= end && (queue_empty || pend_tile != 0xFFFFFFFFu);
         if (starved) {
             const uint64_t leaders = __ballot(r.phase != PH_IDLE && lane % K == 0u);
             const uint32_t nb = (uint32_t)__popcll(leaders);
@@ -986,6 +990,7 @@ static uint32_t default_lanes_per_ray(uint32_t resident_lanes, uint64_t rays)
  * claims are many times the number of waves and the queue balances. */
 struct LaunchPlan {
     uint32_t blocks, lanes_per_ray, tile_rays, resident_lanes;
+    bool direct;
 };
 
 /* March-sample budgets of the launch levels; the last level has none. */
@@ -1023,9 +1028,10 @@ static uint32_t kernel_blocks_per_cu(uint32_t W, int precision, bool margin)
 }
 
 static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, const LyapTuning &tune,
-                              uint32_t W, int precision, bool margin)
+                              uint32_t W, int precision, bool margin, bool direct = false)
 {
     LaunchPlan p;
+    p.direct = direct;
     const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
     uint32_t per_cu = kernel_blocks_per_cu(W, precision, margin);
     if (tune.waves_per_cu && (tune.waves_per_cu + 3u) / 4u < per_cu) per_cu = (tune.waves_per_cu + 3u) / 4u;
@@ -1037,6 +1043,10 @@ static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, const LyapTuni
     if (p.lanes_per_ray == 0) p.lanes_per_ray = 1;
     p.tile_rays = tune.tile_rays ? tune.tile_rays : 64u / p.lanes_per_ray;
     if (p.tile_rays == 0) p.tile_rays = 1;
+    if (direct) {
+        p.lanes_per_ray = 1;
+        p.tile_rays = 64;
+    }
     return p;
 }
 
@@ -1156,7 +1166,8 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
     /* level l reads the rays level l-1 handed over (buffers alternate) */
     ContEntry *out_buf[4] = {bufA, bufB, bufA, nullptr};
     const ContEntry *in_buf[4] = {nullptr, bufA, bufB, bufA};
-    for (uint32_t level = 0; level < 4; ++level) {
+    const uint32_t levels = plan.direct ? 1u : 4u;
+    for (uint32_t level = 0; level < levels; ++level) {
         RenderArgs a{};
         a.n_views = n_views;
         a.total_tiles = total_tiles;
@@ -1166,7 +1177,8 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         a.eval_counter = (unsigned long long *)(res.counters + 2);
         a.wave_log = (unsigned long long *)ctx->debug_wave_log;
         a.level = level;
-        a.budget = kLevelBudget[level];
+        a.budget = plan.direct ? 0u : kLevelBudget[level];
+        a.direct = plan.direct ? 1u : 0u;
         a.resident_lanes = plan.resident_lanes;
         a.cont_in = in_buf[level];
         a.in_counts = level ? counts + (level - 1) * n_views : nullptr;
@@ -1228,6 +1240,8 @@ static int check_mode(lyap_ctx *ctx, const LyapRenderMode *mode_in, LyapRenderMo
         return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "bisect_steps outside 0..64");
     if (mode.lanes_per_ray > 64)
         return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "lanes_per_ray above 64");
+    if (mode.schedule != LYAP_SCHEDULE_BATCHED && mode.schedule != LYAP_SCHEDULE_DIRECT)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown schedule %d", mode.schedule);
     return LYAP_OK;
 }
 
@@ -1279,7 +1293,8 @@ static int render_views(lyap_ctx *ctx, std::vector<ViewDesc> &views, uint32_t W,
     const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
     LyapTuning t = tune;
     if (!t.lanes_per_ray && mode.lanes_per_ray) t.lanes_per_ray = mode.lanes_per_ray;
-    const LaunchPlan plan = plan_launch(ctx, rays, t, W, mode.precision, margin);
+    const LaunchPlan plan = plan_launch(ctx, rays, t, W, mode.precision, margin,
+                                        mode.schedule == LYAP_SCHEDULE_DIRECT);
     RenderResources &res = ctx_resources(ctx);
     uint32_t total_tiles = 0;
     int rc = prepare_views(ctx, res, views.data(), (unsigned)views.size(), plan, W, true, &total_tiles);
@@ -1309,6 +1324,54 @@ int launch_render(lyap_ctx *ctx, LyapRGBA *rgba, LyapPoint *points,
               num_lights, mode);
     const uint32_t W = e.L <= 16 ? 16u : 32u;
     return render_views(ctx, views, W, mode, merged_tuning(ctx, nullptr));
+}
+
+/* ---- launch slots for the server ----------------------------------------- */
+
+struct RenderSlot {
+    RenderResources res;
+};
+
+RenderSlot *render_slot_create() { return new RenderSlot(); }
+
+void render_slot_destroy(lyap_ctx *ctx, RenderSlot *slot)
+{
+    if (!slot) return;
+    release_resources(ctx, slot->res);
+    delete slot;
+}
+
+const uint32_t *render_slot_counters(const RenderSlot *slot) { return slot->res.counters; }
+
+/* Renders rays [ray_begin[i], ray_begin[i] + ray_count[i]) of views[i] for
+ * every i, in that order of priority, with the slot's own resources. The
+ * caller guarantees the slot's previous launch has finished, so nothing here
+ * waits on the device. ray_begin is a multiple of 64 (whole 8x8 blocks). */
+int render_slot_launch(lyap_ctx *ctx, RenderSlot *slot, const LyapView *in,
+                       const uint32_t *ray_begin, const uint32_t *ray_count, unsigned n,
+                       const LyapRenderMode &mode, const LyapTuning &tune)
+{
+    std::vector<ViewDesc> views(n);
+    uint32_t maxL = 1;
+    uint64_t rays = 0;
+    for (unsigned i = 0; i < n; ++i) {
+        int rc = view_from_api(ctx, views[i], in[i], mode);
+        if (rc) return rc;
+        views[i].ray_begin = ray_begin[i];
+        views[i].n_rays = ray_count[i];
+        rays += ray_count[i];
+        if ((uint32_t)views[i].e.L > maxL) maxL = (uint32_t)views[i].e.L;
+    }
+    const uint32_t W = maxL <= 16 ? 16u : 32u;
+    const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
+    LyapTuning t = tune;
+    if (!t.lanes_per_ray && mode.lanes_per_ray) t.lanes_per_ray = mode.lanes_per_ray;
+    const LaunchPlan plan = plan_launch(ctx, rays, t, W, mode.precision, margin,
+                                        mode.schedule == LYAP_SCHEDULE_DIRECT);
+    uint32_t total_tiles = 0;
+    int rc = prepare_views(ctx, slot->res, views.data(), n, plan, W, false, &total_tiles);
+    if (rc) return rc;
+    return enqueue_views(ctx, slot->res, views.data(), n, total_tiles, plan, W, mode.precision);
 }
 
 } /* namespace lyap */
@@ -1440,7 +1503,8 @@ extern "C" int lyap_frame_capture(lyap_ctx *ctx, const LyapView *views, unsigned
     uint64_t rays = 0;
     for (auto &v : vd) rays += v.n_rays;
     const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
-    const LaunchPlan plan = plan_launch(ctx, rays, merged_tuning(ctx, tuning), W, mode.precision, margin);
+    const LaunchPlan plan = plan_launch(ctx, rays, merged_tuning(ctx, tuning), W, mode.precision, margin,
+                                        mode.schedule == LYAP_SCHEDULE_DIRECT);
     lyap_frame *f = new lyap_frame();
     f->ctx = ctx;
     uint32_t total_tiles = 0;

@@ -21,7 +21,7 @@ struct VolumeArgs {
     float *out;
     const float *xyz;          /* point list, or null for the grid */
     uint64_t n;                /* voxels or points */
-    uint32_t nx, ny, nz, pad;
+    uint32_t nx, ny, nz, z0;   /* z0: first z slice of this launch */
     float lo[3], span[3];
     uint32_t *list;            /* FP64 re-evaluation list */
     uint32_t *count;
@@ -40,8 +40,9 @@ __device__ __forceinline__ void voxel_coords(const VolumeArgs &a, uint64_t idx,
         return;
     }
     const uint64_t plane = (uint64_t)a.nx * a.ny;
-    const uint32_t iz = (uint32_t)(idx / plane);
-    const uint32_t rem = (uint32_t)(idx - (uint64_t)iz * plane);
+    const uint32_t lz = (uint32_t)(idx / plane);
+    const uint32_t rem = (uint32_t)(idx - (uint64_t)lz * plane);
+    const uint32_t iz = a.z0 + lz;
     const uint32_t iy = rem / a.nx;
     const uint32_t ix = rem - iy * a.nx;
     x = a.lo[0] + (a.span[0] * (float)ix) / (float)a.nx;
@@ -178,13 +179,23 @@ int launch_volume(lyap_ctx *ctx, float *out, const EvalDesc &e,
                   unsigned ny, unsigned nz, const float *xyz, size_t npoints,
                   const LyapVolumeMode &mode)
 {
+    return launch_volume_slab(ctx, out, e, lo, hi, nx, ny, nz, 0, nz, xyz, npoints, mode);
+}
+
+/* z slices [z_begin, z_end) of the grid; out points at slice z_begin. */
+int launch_volume_slab(lyap_ctx *ctx, float *out, const EvalDesc &e,
+                       const float lo[3], const float hi[3], unsigned nx,
+                       unsigned ny, unsigned nz, unsigned z_begin, unsigned z_end,
+                       const float *xyz, size_t npoints, const LyapVolumeMode &mode)
+{
     VolumeArgs a{};
     a.out = out;
     a.xyz = xyz;
     if (xyz) {
         a.n = npoints;
     } else {
-        a.n = (uint64_t)nx * ny * nz;
+        a.n = (uint64_t)nx * ny * (z_end - z_begin);
+        a.z0 = z_begin;
         a.nx = nx;
         a.ny = ny;
         a.nz = nz;

@@ -56,11 +56,12 @@ LAYOUT_C = r"""
 #define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
 int main(void) {
   S(LyapParams) S(LyapPoint) S(LyapCam) S(LyapRGBA) S(LyapView) S(LyapRequest)
-  S(LyapVolumeMode) S(LyapRenderMode) S(LyapTuning) S(LyapServerConfig) S(LyapResult)
+  S(LyapVolumeMode) S(LyapRenderMode) S(LyapTuning) S(LyapServerConfig) S(LyapResult) S(LyapServerStats)
   O(LyapCam, Q) O(LyapCam, ambient) O(LyapCam, chaosColor) O(LyapView, prm) O(LyapView, seq)
   O(LyapView, rgba) O(LyapView, num_lights) O(LyapRequest, view) O(LyapRequest, vol_prm)
   O(LyapRequest, vol_out) O(LyapRequest, deadline_ms) O(LyapServerConfig, mode) O(LyapServerConfig, vol_mode)
-  O(LyapResult, submit_ms) O(LyapResult, lambda)
+  O(LyapResult, submit_ms) O(LyapResult, lambda) O(LyapServerConfig, slice_ms)
+  O(LyapServerStats, resident_bytes) O(LyapServerStats, device_ms)
   return 0;
 }
 """
@@ -81,7 +82,8 @@ def test_struct_layouts_match_c(tmp_path):
     py = {"LyapParams": lyap.LyapParams, "LyapPoint": lyap.LyapPoint, "LyapCam": lyap.LyapCam,
           "LyapRGBA": lyap.RGBA, "LyapView": lyap.View, "LyapRequest": lyap.Request,
           "LyapVolumeMode": lyap.VolumeMode, "LyapRenderMode": lyap.RenderMode, "LyapTuning": lyap.Tuning,
-          "LyapServerConfig": lyap.ServerConfig, "LyapResult": lyap.Result}
+          "LyapServerConfig": lyap.ServerConfig, "LyapResult": lyap.Result,
+          "LyapServerStats": lyap.ServerStats}
     for name, cls in py.items():
         assert C.sizeof(cls) == c[name][0], name
     assert c["LyapParams"][0] == 56 and c["LyapPoint"][0] == 36 and c["LyapCam"][0] == 224
@@ -92,6 +94,8 @@ def test_struct_layouts_match_c(tmp_path):
               "LyapRequest.vol_prm": lyap.Request.vol_prm, "LyapRequest.vol_out": lyap.Request.vol_out,
               "LyapRequest.deadline_ms": lyap.Request.deadline_ms, "LyapServerConfig.mode": lyap.ServerConfig.mode,
               "LyapServerConfig.vol_mode": lyap.ServerConfig.vol_mode, "LyapResult.submit_ms": lyap.Result.submit_ms,
-              "LyapResult.lambda": lyap.Result.lambda_}
+              "LyapResult.lambda": lyap.Result.lambda_, "LyapServerConfig.slice_ms": lyap.ServerConfig.slice_ms,
+              "LyapServerStats.resident_bytes": lyap.ServerStats.resident_bytes,
+              "LyapServerStats.device_ms": lyap.ServerStats.device_ms}
     for k, f in fields.items():
         assert f.offset == c[k][0], k

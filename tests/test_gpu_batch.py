@@ -137,3 +137,45 @@ def test_batch_errors(ctx):
     with pytest.raises(lyap.LyapError) as e:
         ctx.render_batch(api)
     assert e.value.status == lyap.ERR_INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("prec", [lyap.PREC_PLAIN, lyap.PREC_PRODUCT, lyap.PREC_FP64])
+def test_direct_schedule_same_bytes(ctx, prec):
+    """The reference's launch shape (one lane per ray, no refill) and the
+    batched schedule compute every ray identically."""
+    views, lights, n_lights = small_batch(6, seed=3)
+    ld = device_lights(lights, n_lights)
+    batched = W.c2_mode(precision=prec, fp64_margin=0.0)
+    direct = W.c2_mode(precision=prec, fp64_margin=0.0)
+    direct.schedule = lyap.SCHEDULE_DIRECT
+    for v in views:
+        a = RenderBuffers(v["w"], v["h"])
+        b = RenderBuffers(v["w"], v["h"])
+        a.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, batched)
+        ctx.sync()
+        ea = ctx.last_evals()
+        b.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, direct)
+        ctx.sync()
+        assert ctx.last_evals() == ea
+        same(a.fetch(), b.fetch())
+    # a batch in the direct shape too
+    bufs = alloc(views)
+    ctx.render_batch(as_api(views, bufs, ld, n_lights), direct)
+    ctx.sync()
+    for v, g in zip(views, bufs):
+        one = RenderBuffers(v["w"], v["h"])
+        one.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, batched)
+        ctx.sync()
+        same(g.fetch(), one.fetch())
+
+
+def test_bad_schedule(ctx):
+    views, lights, n_lights = small_batch(1)
+    ld = device_lights(lights, n_lights)
+    m = W.c2_mode(fp64_margin=0.0)
+    m.schedule = 7
+    b = RenderBuffers(8, 8)
+    v = views[0]
+    with pytest.raises(lyap.LyapError) as e:
+        b.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, m)
+    assert e.value.status == lyap.ERR_INVALID_ARGUMENT

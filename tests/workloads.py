@@ -96,3 +96,42 @@ def batch_views(n_views: int = 64, seed: int = 4, sides=(128, 256, 384, 512, 768
                                opaqueThreshold=0.0, lMin=2.0, lMax=4.0)
         views.append(dict(cam=cam, prm=prm, seq=random_sequence(rng), w=side, h=side))
     return views, lights, n_lights
+
+
+def serving_stream(seconds: float = 60.0, rate: float = 200.0, seed: int = 5, settle: int = 100):
+    """The serving workload: Poisson arrivals at `rate` per second for
+    `seconds`. 70% are views (side U{256,512,1024}, camera and sequence as in
+    batch_views), 30% volume blocks (edge U{32,64,128,256} over a random cube
+    inside [2,4]^3 with side U[0.25,2]); accumulation U{200..4000}, settle
+    100. Returns (requests, lights, n_lights); each request is a dict with
+    its arrival time in ms."""
+    rng = np.random.default_rng(seed)
+    _, _, lights, n_lights = lyap.params_init()
+    lyap.lights_recalculate(lights, n_lights)
+    out = []
+    t = 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= seconds:
+            break
+        accum = int(rng.integers(200, 4001))
+        if rng.random() < 0.7:
+            side = int(rng.choice((256, 512, 1024)))
+            radius = float(rng.uniform(3.0, 6.0))
+            d = rng.normal(size=3)
+            d = d / np.linalg.norm(d)
+            eye = tuple(float(CENTER[k] + radius * d[k]) for k in range(3))
+            cam = lyap.cam_look_at(eye, CENTER, math.tan(math.radians(22.5)), side, side)
+            prm = lyap.make_params(settle=settle, accum=accum, stepMethod=1, depth=500.0, jitter=0.0,
+                                   nearMultiplier=1.0, opaqueThreshold=0.0, lMin=2.0, lMax=4.0)
+            out.append(dict(kind="view", t_ms=1000.0 * t, cam=cam, prm=prm, seq=random_sequence(rng),
+                            w=side, h=side))
+        else:
+            edge = int(rng.choice((32, 64, 128, 256)))
+            s = float(rng.uniform(0.25, 2.0))
+            lo = tuple(float(2.0 + rng.uniform(0.0, 2.0 - s)) for _ in range(3))
+            hi = tuple(x + s for x in lo)
+            prm = lyap.make_params(settle=settle, accum=accum)
+            out.append(dict(kind="volume", t_ms=1000.0 * t, prm=prm, seq=random_sequence(rng),
+                            lo=lo, hi=hi, n=edge))
+    return out, lights, n_lights
