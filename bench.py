@@ -65,9 +65,27 @@ def parse():
     p.add_argument("--c4-views", type=int, default=64)
     p.add_argument("--c5-seconds", type=float, default=3.0,
                    help="length of the paced request stream (the full workload is 60)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="first run a plain 1 GiB device copy three times (the reference kernel "
+                        "for scaling HBM counters)")
     p.add_argument("--c5-capacity", type=int, default=120,
                    help="requests offered at once for the served vs plain capacity")
     return p.parse_args()
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (lines 'TRAFFIC <kernel> <bytes per launch>'), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_render_c2.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def flops(prm):
@@ -386,6 +404,16 @@ def main():
     bufs = RenderBuffers(w, h)
     ld = device_lights(lights, n_lights)
 
+    if args.copy_calibration:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("copy calibration: 3 x 1 GiB device copies")
+
     log(f"warmup {args.warmup} frames")
     for _ in range(args.warmup):
         bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
@@ -395,6 +423,7 @@ def main():
     torch.cuda.synchronize()
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ctx.timing(True)
     t0 = time.perf_counter()
     for i in range(args.steps):
         starts[i].record(timer.stream)
@@ -403,13 +432,21 @@ def main():
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    kernel_ms, launches = ctx.timing_collect()
+    ctx.timing(False)
     log(f"timed {args.steps} frames in {wall:.3f} s")
     per = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])   # ms, on the library stream
     ms = float(wall * 1000.0 / args.steps)
 
     rays = w * h
     frame_ms = float(per.mean())
-    achieved = evals_per_frame * flops(prm) / (frame_ms * 1e-3) / 1e12
+    # the render kernel: four level launches per frame; algorithmic FLOP of
+    # a frame over the summed duration of its launches (HIP events on the
+    # library stream around each launch)
+    launch_ms = kernel_ms / max(launches, 1)
+    launches_per_frame = launches / args.steps
+    achieved = evals_per_frame * flops(prm) / launches_per_frame / (launch_ms * 1e-3) / 1e12
+    traffic = profiled_traffic("render_persistent")
     line = {
         "metric": "lambda-evals/s & rays/s on 1 MI355X; p50/p99 latency, served vs plain",
         "value": rays / (ms * 1e-3),
@@ -439,9 +476,15 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
             "flop_per_lambda_eval": flops(prm),
-            "note": "render kernel (four level launches per frame), algorithmic FLOP per frame / frame time",
+            "kernel": "render_persistent",
+            "launches_per_frame": launches_per_frame,
+            "avg_launch_us": launch_ms * 1e3,
+            "flop_per_launch": evals_per_frame * flops(prm) / launches_per_frame,
+            "note": "achieved = algorithmic FLOP per launch (lambda evaluations x (3 settle + 6 accum)) / "
+                    "mean launch duration from HIP events on the library stream; traffic = HBM bytes per "
+                    "launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy",
         },
     }
     which = [x for x in args.workloads.split(",") if x]

@@ -223,6 +223,14 @@ int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_points,
  * clock, end clock}; NULL switches it off. No reference counterpart. */
 int lyap_debug_wave_log(lyap_ctx *ctx, void *d_log);
 
+/* Kernel timing: while on, every render kernel launch on the context's
+ * stream is bracketed by a pair of HIP events. lyap_timing_collect waits for
+ * the stream, returns the summed kernel milliseconds and the number of
+ * launches since timing was switched on or last collected, and starts over.
+ * No reference counterpart (measurement only). */
+int lyap_timing_enable(lyap_ctx *ctx, int on);
+int lyap_timing_collect(lyap_ctx *ctx, double *kernel_ms, uint64_t *launches);
+
 /* Total algorithmic lambda evaluations of the last render/batch call
  * (march samples through the first opaque one, refinement, normal). Waits
  * for that call to finish. */

@@ -226,6 +226,8 @@ _SIGS = {
     "lyap_calc_render": (C.c_int, [_P, _P, _P, C.POINTER(LyapCam), C.POINTER(LyapParams), C.POINTER(C.c_int32), _P, C.c_uint, C.c_uint, C.c_uint, _P, _P, _P, C.POINTER(RenderMode)]),
     "lyap_last_evals": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "lyap_debug_wave_log": (C.c_int, [_P, _P]),
+    "lyap_timing_enable": (C.c_int, [_P, C.c_int]),
+    "lyap_timing_collect": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "lyap_render_batch": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning)]),
     "lyap_autotune": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning)]),
     "lyap_frame_capture": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning), C.POINTER(_P)]),
@@ -452,6 +454,17 @@ class Context:
                                        depth_ptr or None, lambda_ptr or None, evals_ptr or None,
                                        C.byref(mode) if mode is not None else None)
         _check(rc, self.h, "lyap_calc_render")
+
+    def timing(self, on: bool):
+        """Brackets every render kernel launch with HIP events while on."""
+        _check(self.lib.lyap_timing_enable(self.h, 1 if on else 0), self.h, "lyap_timing_enable")
+
+    def timing_collect(self):
+        """(summed kernel ms, launches) since timing was switched on or last collected."""
+        ms = C.c_double()
+        n = C.c_uint64()
+        _check(self.lib.lyap_timing_collect(self.h, C.byref(ms), C.byref(n)), self.h, "lyap_timing_collect")
+        return ms.value, n.value
 
     def last_evals(self) -> int:
         c = C.c_uint64()

@@ -77,6 +77,19 @@ static int check_params(lyap_ctx *ctx, const LyapParams *prm, const int32_t *seq
     return LYAP_OK;
 }
 
+int timing_mark(lyap_ctx *ctx)
+{
+    if (!ctx->timing) return LYAP_OK;
+    if (ctx->timing_used == ctx->timing_events.size()) {
+        hipEvent_t e;
+        LYAP_HIP(ctx, hipEventCreate(&e));
+        ctx->timing_events.push_back(e);
+    }
+    LYAP_HIP(ctx, hipEventRecord(ctx->timing_events[ctx->timing_used], ctx->stream));
+    ctx->timing_used++;
+    return LYAP_OK;
+}
+
 } /* namespace lyap */
 
 using namespace lyap;
@@ -141,6 +154,7 @@ void lyap_destroy(lyap_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_render_resources(ctx);
+    for (hipEvent_t e : ctx->timing_events) (void)hipEventDestroy(e);
     for (auto &kv : ctx->allocations) (void)hipFree(kv.first);
     ctx->allocations.clear();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -237,6 +251,30 @@ int lyap_last_reevaluated(lyap_ctx *ctx, uint64_t *count)
     LYAP_HIP(ctx, hipMemcpyAsync(&c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
     LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *count = c;
+    return LYAP_OK;
+}
+
+int lyap_timing_enable(lyap_ctx *ctx, int on)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    ctx->timing = on != 0;
+    ctx->timing_used = 0;
+    return LYAP_OK;
+}
+
+int lyap_timing_collect(lyap_ctx *ctx, double *kernel_ms, uint64_t *launches)
+{
+    if (!ctx || !kernel_ms || !launches) return LYAP_ERR_INVALID_ARGUMENT;
+    LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double total = 0.0;
+    for (size_t i = 0; i + 1 < ctx->timing_used; i += 2) {
+        float ms = 0.0f;
+        LYAP_HIP(ctx, hipEventElapsedTime(&ms, ctx->timing_events[i], ctx->timing_events[i + 1]));
+        total += ms;
+    }
+    *kernel_ms = total;
+    *launches = ctx->timing_used / 2;
+    ctx->timing_used = 0;
     return LYAP_OK;
 }
 

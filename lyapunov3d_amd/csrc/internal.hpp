@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "lyap.h"
 #include "lyap_device.hpp"
@@ -33,6 +34,11 @@ struct lyap_ctx {
     LyapTuning tuned{};                /* autotuner's choice, zero = unset */
     void *debug_wave_log = nullptr;    /* per-wave render record, when set */
     int num_cus = 0;
+
+    /* kernel timing: event pairs around render launches, reused */
+    bool timing = false;
+    std::vector<hipEvent_t> timing_events;
+    size_t timing_used = 0;            /* events recorded since the last collect */
 };
 
 namespace lyap {
@@ -44,6 +50,10 @@ int ctx_alloc(lyap_ctx *ctx, void **ptr, size_t bytes);
 void ctx_free(lyap_ctx *ctx, void *ptr);
 
 int ensure_reeval_list(lyap_ctx *ctx, size_t entries);
+
+/* Records the next timing event on the context's stream (no-op when timing
+ * is off). */
+int timing_mark(lyap_ctx *ctx);
 
 /* Launchers in the kernel translation units. */
 int launch_zero_u32(lyap_ctx *ctx, uint32_t *p, uint32_t n);

@@ -1188,6 +1188,8 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         if (level == 0 && blocks > needed0) blocks = (uint32_t)needed0;
         if (blocks == 0) blocks = 1;
         if (blocks > cus * 8u) blocks = cus * 8u;
+        rc = timing_mark(ctx);
+        if (rc) return rc;
 #define LYAP_LAUNCH(WW, PP, MM)                                                       \
     hipLaunchKernelGGL((render_persistent<WW, PP, MM>), dim3(blocks), dim3(256), 0, ctx->stream, \
                        (const ViewDesc *)res.view_table, a)
@@ -1204,6 +1206,8 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         }
 #undef LYAP_LAUNCH
         LYAP_HIP(ctx, hipGetLastError());
+        rc = timing_mark(ctx);
+        if (rc) return rc;
     }
     ctx->last_counters = res.counters;
     return LYAP_OK;
@@ -1516,7 +1520,11 @@ extern "C" int lyap_frame_capture(lyap_ctx *ctx, const LyapView *views, unsigned
         if (e != hipSuccess) {
             rc = fail(ctx, LYAP_ERR_DEVICE, "stream capture: %s", hipGetErrorString(e));
         } else {
+            /* no timing events inside a graph */
+            const bool timing = ctx->timing;
+            ctx->timing = false;
             rc = enqueue_views(ctx, f->res, vd.data(), n_views, total_tiles, plan, W, mode.precision);
+            ctx->timing = timing;
             hipGraph_t g = nullptr;
             e = hipStreamEndCapture(ctx->stream, &g);
             if (rc == LYAP_OK && e != hipSuccess)
