@@ -504,6 +504,69 @@ __device__ __forceinline__ void product_block(float &u, ProductState &st,
     renorm(st);
 }
 
+/* Calls f.template go<B>() with B the block length Bv, one of (W/2, W]. A
+ * full block then runs exactly B unguarded steps. Bv comes from the view
+ * descriptor and is the same for every lane of the wave, so the choice is a
+ * scalar branch. */
+template <int W, int B, class F>
+__device__ __forceinline__ void with_block_len(uint32_t Bv, F &f)
+{
+    if constexpr (B > W / 2) {
+        if (Bv == (uint32_t)B) f.template go<B>();
+        else with_block_len<W, B - 1>(Bv, f);
+    }
+}
+
+/* n full settle blocks of B steps. */
+template <int W>
+struct SettleFull {
+    float &u;
+    const float (&b)[W];
+    uint32_t n;
+    template <int B>
+    __device__ __forceinline__ void go()
+    {
+        for (uint32_t k = 0; k < n; ++k) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) u = umap(u, b[j]);
+        }
+    }
+};
+
+/* n full product blocks of B steps, with the same underflow redo as
+ * product_block. */
+template <int W>
+struct ProductFull {
+    float &u;
+    ProductState &st;
+    const float (&b)[W];
+    uint32_t n;
+    template <int B>
+    __device__ __forceinline__ void go()
+    {
+        for (uint32_t k = 0; k < n; ++k) {
+            const float u0 = u;
+            const ProductState s0 = st;
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                u = umap(u, b[j]);
+                st.M *= __builtin_fabsf(u);
+            }
+            if (st.M < kUnderflowGuard) {
+                float w = u0;
+                st = s0;
+#pragma unroll
+                for (int j = 0; j < B; ++j) {
+                    w = umap(w, b[j]);
+                    st.M *= __builtin_fabsf(w);
+                    renorm(st);
+                }
+            }
+            renorm(st);
+        }
+    }
+};
+
 /* FP32 product form, any period L <= W. */
 template <int W>
 __device__ __forceinline__ float lambda_product_w(float x, float y, float z,
@@ -512,12 +575,18 @@ __device__ __forceinline__ float lambda_product_w(float x, float y, float z,
     float b[W];
     load_half_r<W>(b, x, y, z, e);
     float u = 0.0f;
-    for (uint32_t k = 0; k < e.set_blocks; ++k) settle_block<W, true>(u, b, 0, e.B);
+    {
+        SettleFull<W> f{u, b, e.set_blocks};
+        with_block_len<W, W>(e.B, f);
+    }
     settle_block<W, false>(u, b, 0, e.set_rem);
     if (__builtin_fabsf(u) < 2e-8f) return 0.0f / (float)e.accum;   /* kernel.cu:138 */
     ProductState st{1.0f, 0};
     product_block<W, false>(u, st, b, e.head_lo, e.head_hi);
-    for (uint32_t k = 0; k < e.acc_blocks; ++k) product_block<W, true>(u, st, b, 0, e.B);
+    {
+        ProductFull<W> f{u, st, b, e.acc_blocks};
+        with_block_len<W, W>(e.B, f);
+    }
     product_block<W, false>(u, st, b, 0, e.acc_rem);
     float lg[4];
     lg[0] = __builtin_amdgcn_logf(__builtin_fabsf(x));
@@ -541,6 +610,26 @@ __device__ __forceinline__ void plain_block(float &v, float &s, const float (&r)
     }
 }
 
+/* n full blocks of B steps of the x-form, accumulating when ACC. */
+template <int W, bool ACC>
+struct PlainFull {
+    float &v;
+    float &s;
+    const float (&r)[W];
+    uint32_t n;
+    template <int B>
+    __device__ __forceinline__ void go()
+    {
+        for (uint32_t k = 0; k < n; ++k) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                v = r[j] * __builtin_fmaf(-v, v, v);
+                if (ACC) s += __builtin_amdgcn_logf(__builtin_fabsf(__builtin_fmaf(-2.0f * r[j], v, r[j])));
+            }
+        }
+    }
+};
+
 template <int W>
 __device__ __forceinline__ float lambda_plain_w(float x, float y, float z,
                                                 const EvalDesc &e)
@@ -549,11 +638,17 @@ __device__ __forceinline__ float lambda_plain_w(float x, float y, float z,
 #pragma unroll
     for (int j = 0; j < W; ++j) r[j] = pick(e.wletters[j], x, y, z, e.d);
     float v = 0.5f, s = 0.0f;
-    for (uint32_t k = 0; k < e.set_blocks; ++k) plain_block<W, true>(v, s, r, 0, e.B, false);
+    {
+        PlainFull<W, false> f{v, s, r, e.set_blocks};
+        with_block_len<W, W>(e.B, f);
+    }
     plain_block<W, false>(v, s, r, 0, e.set_rem, false);
     if (v == 0.5f) return 0.0f / (float)e.accum;
     plain_block<W, false>(v, s, r, e.head_lo, e.head_hi, true);
-    for (uint32_t k = 0; k < e.acc_blocks; ++k) plain_block<W, true>(v, s, r, 0, e.B, true);
+    {
+        PlainFull<W, true> f{v, s, r, e.acc_blocks};
+        with_block_len<W, W>(e.B, f);
+    }
     plain_block<W, false>(v, s, r, 0, e.acc_rem, true);
     const float lam = s * kLn2f / (float)e.accum;
     return __builtin_isfinite(lam) ? lam : __builtin_nanf("");
@@ -581,6 +676,31 @@ __device__ __forceinline__ void fp64_block(double &v, ProductState64 &st,
     if (ACC) renorm64(st);
 }
 
+/* n full FP64 blocks of B steps. */
+template <int W, bool ACC>
+struct Fp64Full {
+    double &v;
+    ProductState64 &st;
+    const double (&r)[W];
+    const double (&m2r)[W];
+    uint32_t n;
+    template <int B>
+    __device__ __forceinline__ void go()
+    {
+        for (uint32_t k = 0; k < n; ++k) {
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                v = (r[j] * v) * (1.0 - v);
+                if (ACC) {
+                    st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
+                    if (W > 16 && j == 15) renorm64(st);
+                }
+            }
+            if (ACC) renorm64(st);
+        }
+    }
+};
+
 template <int W>
 __device__ __forceinline__ double lambda_fp64_w(float xf, float yf, float zf,
                                                 const EvalDesc &e)
@@ -594,12 +714,18 @@ __device__ __forceinline__ double lambda_fp64_w(float xf, float yf, float zf,
     }
     double v = 0.5;
     ProductState64 st{1.0, 0};
-    for (uint32_t k = 0; k < e.set_blocks; ++k) fp64_block<W, true, false>(v, st, r, m2r, 0, e.B);
+    {
+        Fp64Full<W, false> f{v, st, r, m2r, e.set_blocks};
+        with_block_len<W, W>(e.B, f);
+    }
     fp64_block<W, false, false>(v, st, r, m2r, 0, e.set_rem);
     const double off = v - 0.5;
     if (!(off <= -1e-8 || off >= 1e-8)) return 0.0 / (double)e.accum;
     fp64_block<W, false, true>(v, st, r, m2r, e.head_lo, e.head_hi);
-    for (uint32_t k = 0; k < e.acc_blocks; ++k) fp64_block<W, true, true>(v, st, r, m2r, 0, e.B);
+    {
+        Fp64Full<W, true> f{v, st, r, m2r, e.acc_blocks};
+        with_block_len<W, W>(e.B, f);
+    }
     fp64_block<W, false, true>(v, st, r, m2r, 0, e.acc_rem);
     const double lam = ((double)st.E * kLn2 + log(st.M)) / (double)e.accum;
     return __builtin_isfinite(lam) ? lam : __builtin_nan("");
