@@ -218,9 +218,11 @@ int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_points,
                      float *d_depth, float *d_lambda, uint32_t *d_evals,
                      const LyapRenderMode *mode);
 
-/* Diagnostic: a device buffer of 4 uint64 per wave of the persistent render
- * grid that later render launches fill with {rounds, rays started, start
- * clock, end clock}; NULL switches it off. No reference counterpart. */
+/* Diagnostic: a device buffer of 8 uint64 per wave slot, 8192 slots per
+ * launch level, 4 levels (2 MiB), that later render launches fill with
+ * {rounds, rays started, start clock, end clock, clocks evaluating, clocks
+ * refilling, clocks combining, 0}; NULL switches it off. No reference
+ * counterpart. */
 int lyap_debug_wave_log(lyap_ctx *ctx, void *d_log);
 
 /* Kernel timing: while on, every render kernel launch on the context's
@@ -261,7 +263,7 @@ typedef struct LyapTuning {
     uint32_t tile_rays;        /* rays handed out per work-queue claim */
     uint32_t waves_per_cu;     /* resident waves requested per CU */
     uint32_t lanes_per_ray;    /* lanes sharing one ray's march */
-    uint32_t reserved;
+    uint32_t slots;            /* samples each lane evaluates per round, 1..4 */
 } LyapTuning;
 
 /* Renders n_views views in one persistent launch fed by a device work queue

@@ -140,7 +140,7 @@ class View(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("tile_rays", C.c_uint32), ("waves_per_cu", C.c_uint32),
-        ("lanes_per_ray", C.c_uint32), ("reserved", C.c_uint32),
+        ("lanes_per_ray", C.c_uint32), ("slots", C.c_uint32),
     ]
 
 

@@ -83,6 +83,9 @@ struct EvalDesc {
      * (positions [head_lo, head_hi)), runs acc_blocks blocks, then positions
      * [0, acc_rem). */
     int32_t wletters[LYAP_MAX_SEQ];
+    /* the same as bit masks: bit j of wmask[k] is set when wletters[j] == k,
+     * so a kernel builds its coefficients from one 16-byte scalar load */
+    uint32_t wmask[4];
     uint32_t W, B;
     uint32_t set_blocks, set_rem;
     uint32_t head_lo, head_hi;
@@ -146,7 +149,11 @@ inline void rebuild_blocks(EvalDesc *e, uint32_t W)
     const uint32_t L = (uint32_t)e->L;
     e->W = W;
     e->B = L * (W / L);
-    for (uint32_t j = 0; j < (uint32_t)LYAP_MAX_SEQ; ++j) e->wletters[j] = e->letters[j % L];
+    for (uint32_t k = 0; k < 4; ++k) e->wmask[k] = 0;
+    for (uint32_t j = 0; j < (uint32_t)LYAP_MAX_SEQ; ++j) {
+        e->wletters[j] = e->letters[j % L];
+        e->wmask[e->wletters[j] & 3] |= 1u << j;
+    }
     e->set_blocks = e->settle / e->B;
     e->set_rem = e->settle % e->B;
     uint32_t left = e->accum;
@@ -171,6 +178,14 @@ __device__ __forceinline__ double pickd(int s, double x, double y, double z,
                                         double d)
 {
     return s == 0 ? x : (s == 1 ? y : (s == 2 ? z : d));
+}
+
+/* Lane's value of the letter at window position j, from the letter masks
+ * (uniform); one scalar load for the whole window. */
+template <typename T>
+__device__ __forceinline__ T pick_mask(const uint32_t (&m)[4], int j, T x, T y, T z, T d)
+{
+    return ((m[0] >> j) & 1u) ? x : (((m[1] >> j) & 1u) ? y : (((m[2] >> j) & 1u) ? z : d));
 }
 
 /* ---- plain FP32: one log2 per accumulated step ------------------------- */
@@ -460,8 +475,11 @@ __device__ __forceinline__ void load_half_r(float (&b)[W], float x, float y,
                                             float z, const EvalDesc &e)
 {
     const float hx = 0.5f * x, hy = 0.5f * y, hz = 0.5f * z, hd = 0.5f * e.d;
+    uint32_t m[4];
 #pragma unroll
-    for (int j = 0; j < W; ++j) b[j] = pick(e.wletters[j], hx, hy, hz, hd);
+    for (int k = 0; k < 4; ++k) m[k] = e.wmask[k];
+#pragma unroll
+    for (int j = 0; j < W; ++j) b[j] = pick_mask<float>(m, j, hx, hy, hz, hd);
 }
 
 template <int W, bool FULL>
@@ -635,8 +653,11 @@ __device__ __forceinline__ float lambda_plain_w(float x, float y, float z,
                                                 const EvalDesc &e)
 {
     float r[W];
+    uint32_t m[4];
 #pragma unroll
-    for (int j = 0; j < W; ++j) r[j] = pick(e.wletters[j], x, y, z, e.d);
+    for (int k = 0; k < 4; ++k) m[k] = e.wmask[k];
+#pragma unroll
+    for (int j = 0; j < W; ++j) r[j] = pick_mask<float>(m, j, x, y, z, e.d);
     float v = 0.5f, s = 0.0f;
     {
         PlainFull<W, false> f{v, s, r, e.set_blocks};
@@ -707,9 +728,12 @@ __device__ __forceinline__ double lambda_fp64_w(float xf, float yf, float zf,
 {
     double r[W], m2r[W];
     const double x = xf, y = yf, z = zf, d = e.d;
+    uint32_t m[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m[k] = e.wmask[k];
 #pragma unroll
     for (int j = 0; j < W; ++j) {
-        r[j] = pickd(e.wletters[j], x, y, z, d);
+        r[j] = pick_mask<double>(m, j, x, y, z, d);
         m2r[j] = -2.0 * r[j];
     }
     double v = 0.5;

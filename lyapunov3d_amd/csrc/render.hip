@@ -74,9 +74,11 @@ struct RenderArgs {
     uint32_t total_tiles;
     uint32_t tile_rays;
     uint32_t lanes_per_ray;    /* lanes marching one ray while rays remain */
+    uint32_t slots;            /* samples each lane evaluates per round (1..4) */
     uint32_t *tile_counter;
     unsigned long long *eval_counter;
-    /* optional per-wave record {rounds, rays, start clock, end clock} */
+    /* optional per-wave record {rounds, rays, start clock, end clock,
+     * clocks in evaluation, clocks in refill/regroup, clocks in combine} */
     unsigned long long *wave_log;
     /* Launch level: 0 takes rays from pixel tiles; a later level takes the
      * rays the previous launch handed over. A ray that has taken `budget`
@@ -280,50 +282,77 @@ __device__ __forceinline__ Ray shuffle_ray(const Ray &r, int src)
     return o;
 }
 
+/* The view's output pointers, read together (one wait for all of them
+ * rather than one per optional output). */
+struct OutPtrs {
+    LyapRGBA *rgba;
+    LyapPoint *points;
+    float *depth;
+    float *lambda;
+    uint32_t *evals;
+    uint32_t present;      /* bit 0 points, 1 depth, 2 lambda, 3 evals */
+};
+
+__device__ __forceinline__ OutPtrs out_ptrs(const ViewDesc &v)
+{
+    OutPtrs o;
+    o.rgba = v.rgba;
+    o.points = v.points;
+    o.depth = v.depth;
+    o.lambda = v.lambda;
+    o.evals = v.evals;
+    o.present = (o.points ? 1u : 0u) | (o.depth ? 2u : 0u) | (o.lambda ? 4u : 0u) |
+                (o.evals ? 8u : 0u);
+    return o;
+}
+
 __device__ __forceinline__ void write_miss(const ViewDesc &v, const Ray &r)
 {
-    if (v.points) {
+    const OutPtrs o = out_ptrs(v);
+    if (o.present & 1u) {
         LyapPoint p;
         p.P = {0.f, 0.f, 0.f};
         p.N = {0.f, 0.f, 0.f};
         p.a = NAN;
         p.c = 0.f;
         p.l = NAN;
-        v.points[r.pix] = p;
+        o.points[r.pix] = p;
     }
-    if (v.depth) v.depth[r.pix] = NAN;
-    if (v.lambda) v.lambda[r.pix] = NAN;
-    if (v.evals) v.evals[r.pix] = r.evals;
+    if (o.present & 2u) o.depth[r.pix] = NAN;
+    if (o.present & 4u) o.lambda[r.pix] = NAN;
+    if (o.present & 8u) o.evals[r.pix] = r.evals;
     LyapRGBA z = {0, 0, 0, 0};
-    v.rgba[r.pix] = z;
+    o.rgba[r.pix] = z;
 }
 
 __device__ void write_hit(const ViewDesc &v, const Ray &r, F3 N)
 {
-    if (v.points) {
+    const OutPtrs o = out_ptrs(v);
+    if (o.present & 1u) {
         LyapPoint p;
         p.P = {r.P.x, r.P.y, r.P.z};
         p.N = {N.x, N.y, N.z};
         p.a = r.a;
         p.c = r.c;
         p.l = r.l;
-        v.points[r.pix] = p;
+        o.points[r.pix] = p;
     }
-    if (v.depth) v.depth[r.pix] = r.t;
-    if (v.lambda) v.lambda[r.pix] = r.l;
-    if (v.evals) v.evals[r.pix] = r.evals;
+    if (o.present & 2u) o.depth[r.pix] = r.t;
+    if (o.present & 4u) o.lambda[r.pix] = r.l;
+    if (o.present & 8u) o.evals[r.pix] = r.evals;
     const C4 cc = shade(r.P, N, r.a, r.c, v.cam, v.lights, v.num_lights);
-    LyapRGBA o;
-    o.r = to_u8(cc.x);
-    o.g = to_u8(cc.y);
-    o.b = to_u8(cc.z);
-    o.a = to_u8(cc.w);
-    v.rgba[r.pix] = o;
+    LyapRGBA px;
+    px.r = to_u8(cc.x);
+    px.g = to_u8(cc.y);
+    px.b = to_u8(cc.z);
+    px.a = to_u8(cc.w);
+    o.rgba[r.pix] = px;
 }
 
-/* Starts ray `id` of view v; returns false (after writing a miss, or nothing
- * for a padding ray past the image edge) when no evaluation is needed. */
-__device__ bool start_ray(const ViewDesc &v, uint32_t id, Ray &r)
+/* Starts ray `id` of view v; returns false (after the leader wrote a miss,
+ * or nothing for a padding ray past the image edge) when no evaluation is
+ * needed. Every lane of the ray's group runs it and gets the same state. */
+__device__ bool start_ray(const ViewDesc &v, uint32_t id, Ray &r, bool leader)
 {
     id += v.ray_begin;
     const uint32_t blk = id >> 6, in = id & 63u;
@@ -339,7 +368,7 @@ __device__ bool start_ray(const ViewDesc &v, uint32_t id, Ray &r)
     r.V = V;
     float t0, t1;
     if (!clip_box(C, V, v.prm.lMin, v.prm.lMax, t0, t1)) {
-        write_miss(v, r);
+        if (leader) write_miss(v, r);
         return false;
     }
     r.t = t0;
@@ -463,7 +492,8 @@ __device__ __forceinline__ uint32_t bisect_levels(uint32_t K, uint32_t left)
     return m < left ? m : left;
 }
 
-/* Point lane `rank` of the group evaluates this trip; false when it has
+/* Point `rank` of the group's K samples this trip (a group of lanes with
+ * several slots each takes K = lanes x slots samples); false when it has
  * nothing to evaluate. Positions repeat the sequential method's float
  * operations exactly. */
 __device__ __forceinline__ bool sample_point(const ViewDesc &v, const Ray &r,
@@ -578,13 +608,31 @@ __device__ __forceinline__ int march_consume(const ViewDesc &v, Ray &r, float l,
     return 1;
 }
 
-/* Combines the group's lambdas (lane gbase + j holds the j-th point) and
- * advances the replicated state in the sequential method's order. All lanes
- * run the same uniform loops so every shuffle sees the whole wave. Returns
- * true when this lane's ray finished this trip. */
-__device__ bool combine(const ViewDesc &v, Ray &r, float lam, bool valid,
-                        float tsamp, F3 psamp, uint32_t gbase, uint32_t K,
-                        bool busy, bool leader)
+/* Samples of one round, per wave, in LDS: sample j of the group whose
+ * first lane is gbase sits at index gbase * slots + j. */
+constexpr uint32_t kMaxSlots = 4;
+
+struct RoundStash {
+    float lam[64 * kMaxSlots];
+    float t[64 * kMaxSlots];
+    float x[64 * kMaxSlots], y[64 * kMaxSlots], z[64 * kMaxSlots];
+    uint32_t ok[64 * kMaxSlots];
+};
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Combines the group's K samples (read from the round's stash at
+ * base + j) and advances the replicated ray state in the sequential
+ * method's order. `evm` has bit j set when sample j stops a march (opaque,
+ * invalid, near/far toggle, jitter). All lanes run the same uniform loops.
+ * Returns true when this lane's ray finished this round. */
+__device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStash &st,
+                        uint32_t base, uint32_t K, bool busy, bool leader)
 {
     const int ph = busy ? r.phase : PH_IDLE;
     const float thr = v.prm.opaqueThreshold;
@@ -593,45 +641,40 @@ __device__ bool combine(const ViewDesc &v, Ray &r, float lam, bool valid,
 
     /* march: the group's samples j = 0..K-1 were taken at the positions the
      * sequential method would reach if none of them stopped it. Find the
-     * first that does (opaque, past the exit, or a change of step length),
-     * account the transparent ones before it in order, then consume it. */
+     * first that does, account the transparent ones before it in order, then
+     * consume it. */
     if (__ballot(ph == PH_MARCH) != 0) {
-        const uint32_t rank = __lane_id() - gbase;
         const bool marching = ph == PH_MARCH;
-        bool ev = true;
-        if (marching && valid) {
-            const LyapParams &prm = v.prm;
-            const bool near_ = (r.flags & 1u) != 0;
-            const bool toggles = (r.index + rank) > 0 && r.Ndt != r.Fdt &&
-                                 ((lam <= prm.nearThreshold) != near_);
-            ev = !(lam > prm.opaqueThreshold) || toggles || prm.jitter != 0.0f;
-        }
-        const uint64_t evm = __ballot(ev);
-        const uint64_t gmask = (K >= 64u) ? ~0ull : (((1ull << K) - 1ull) << gbase);
-        const uint64_t mine = (evm & gmask) >> gbase;
-        const uint32_t e = mine ? (uint32_t)__ffsll((unsigned long long)mine) - 1u : K;
-        /* transparent samples 0..e-1, summed in order (kernel.cu:370-375) */
-        for (uint32_t j = 0; __ballot(marching && j < e) != 0; ++j) {
-            const float lj = shf(lam, (int)(gbase + j));
-            if (marching && j < e) {
-                if (r.index > 0) {
-                    const bool to_c = lj > v.prm.chaosThreshold;
-                    const bool to_a = !to_c && lj > v.prm.opaqueThreshold;
-                    r.c += to_c ? lj : 0.0f;
-                    r.a += to_a ? lj : 0.0f;
+        const uint32_t e = evm ? (uint32_t)__ffsll((unsigned long long)evm) - 1u : K;
+        /* transparent samples 0..e-1, summed in order (kernel.cu:370-375);
+         * four loads issued ahead of their sequential adds */
+        for (uint32_t j0 = 0; __ballot(marching && j0 < e) != 0; j0 += 4u) {
+            float l4[4];
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) l4[q] = st.lam[min(base + j0 + q, 64u * kMaxSlots - 1u)];
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) {
+                const float lj = l4[q];
+                if (marching && j0 + q < e) {
+                    if (r.index > 0) {
+                        const bool to_c = lj > v.prm.chaosThreshold;
+                        const bool to_a = !to_c && lj > v.prm.opaqueThreshold;
+                        r.c += to_c ? lj : 0.0f;
+                        r.a += to_a ? lj : 0.0f;
+                    }
+                    ++r.evals;
+                    ++r.launch_samples;
+                    ++r.index;
+                    r.l = lj;
                 }
-                ++r.evals;
-                ++r.launch_samples;
-                ++r.index;
-                r.l = lj;
             }
         }
         /* position and lambda of sample e */
-        const uint32_t src = gbase + (e < K ? e : K - 1u);
-        const float te = shf(tsamp, (int)src);
-        const F3 Pe = {shf(psamp.x, (int)src), shf(psamp.y, (int)src), shf(psamp.z, (int)src)};
-        const float le = shf(lam, (int)src);
-        const uint32_t ve = shu(valid ? 1u : 0u, (int)src);
+        const uint32_t src = min(base + (e < K ? e : K - 1u), 64u * kMaxSlots - 1u);
+        const float te = st.t[src];
+        const F3 Pe = {st.x[src], st.y[src], st.z[src]};
+        const float le = st.lam[src];
+        const uint32_t ve = st.ok[src];
         if (marching) {
             if (e > 0) {
                 r.last_step = r.dt;
@@ -662,13 +705,13 @@ __device__ bool combine(const ViewDesc &v, Ray &r, float lam, bool valid,
         }
     }
 
-    /* bisection: follow the actual path through this trip's tree levels */
+    /* bisection: follow the actual path through this round's tree levels */
     if (__ballot(ph == PH_REFINE && bisect) != 0) {
         const uint32_t m = (ph == PH_REFINE && bisect)
                                ? bisect_levels(K, (uint32_t)v.bisect_steps - r.k) : 0u;
         uint32_t node = 0;
         for (uint32_t i = 0; __ballot(i < m) != 0; ++i) {
-            const float ln = shf(lam, (int)(gbase + node));
+            const float ln = st.lam[min(base + node, 64u * kMaxSlots - 1u)];
             if (i < m) {
                 const float mid = r.lo + (r.hi - r.lo) * 0.5f;
                 ++r.evals;
@@ -685,9 +728,9 @@ __device__ bool combine(const ViewDesc &v, Ray &r, float lam, bool valid,
         }
     }
 
-    /* reversing walk (kernel.cu:420-441): one sample, by the first lane */
+    /* reversing walk (kernel.cu:420-441): one sample, the group's first */
     if (__ballot(ph == PH_REFINE && !bisect) != 0) {
-        const float l0 = shf(lam, (int)gbase);
+        const float l0 = st.lam[min(base, 64u * kMaxSlots - 1u)];
         if (ph == PH_REFINE && !bisect) {
             r.t += r.Qdt;
             r.P = add(r.P, r.QdV);
@@ -713,9 +756,13 @@ __device__ bool combine(const ViewDesc &v, Ray &r, float lam, bool valid,
     /* normal: the six samples in order (kernel.cu:456-473) */
     if (__ballot(ph == PH_NORMAL) != 0) {
         const uint32_t n6 = K < 6u ? K : 6u;
-        for (uint32_t j = 0; j < n6; ++j) {
-            const float lj = shf(lam, (int)(gbase + j));
-            if (ph == PH_NORMAL && r.k < 6u) {
+        float l6[6];
+#pragma unroll
+        for (uint32_t j = 0; j < 6u; ++j) l6[j] = st.lam[min(base + j, 64u * kMaxSlots - 1u)];
+#pragma unroll
+        for (uint32_t j = 0; j < 6u; ++j) {
+            const float lj = l6[j];
+            if (j < n6 && ph == PH_NORMAL && r.k < 6u) {
                 const uint32_t n = r.k;
                 const float sl = (n & 1u) ? lj : -lj;
                 if (n < 2) r.gx = (n == 0) ? sl : r.gx + sl;
@@ -778,6 +825,7 @@ template <int W, int PREC, bool MARGIN>
 __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restrict__ views,
                                                          RenderArgs a)
 {
+    __shared__ RoundStash stash[4];   /* each wave's samples of the round */
     const uint32_t lane = __lane_id();
     Ray r;
     r.phase = PH_IDLE;
@@ -785,21 +833,27 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
     r.evals = 0;
     uint32_t done_evals = 0;
 
+    /* samples per lane and round; a group of K lanes takes K * S samples */
+    const uint32_t S = uniform(a.slots >= 1u && a.slots <= kMaxSlots ? a.slots : 1u);
     uint32_t K0 = a.lanes_per_ray;    /* lanes per ray while rays remain */
     uint32_t tile = a.tile_rays;
     if (a.level > 0) {
-        /* size the groups to the number of rays handed over */
+        /* size the groups to the number of rays handed over: samples per
+         * ray and round as lanes would be if each took one */
         uint32_t total = 0;
         for (uint32_t v = 0; v < a.n_views; ++v)
             total += uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
         if (total == 0) return;
         K0 = 1;
         while (K0 < 64u && (uint64_t)a.resident_lanes >= (uint64_t)total * (2u * K0)) K0 *= 2;
+        K0 = K0 / S > 0u ? K0 / S : 1u;
         tile = uniform(64u / K0);
     }
+    if (K0 * S > 64u) K0 = 64u / S;
     K0 = uniform(K0);
     uint32_t K = K0;                  /* lanes per ray; wave-uniform */
     const unsigned long long t_start = a.wave_log ? clock64() : 0ull;
+    unsigned long long c_eval = 0, c_refill = 0, c_combine = 0, c_mark = t_start;
     uint32_t rounds = 0, rays_started = 0;
     uint32_t cur_view = 0xFFFFFFFFu;  /* view the lanes work on */
     uint32_t pos = 0, end = 0;        /* unstarted rays of the current tile */
@@ -807,6 +861,11 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
     bool queue_empty = false;
 
     for (;;) {
+        if (a.wave_log) {
+            const unsigned long long now = clock64();
+            c_combine += now - c_mark;
+            c_mark = now;
+        }
         /* ---- refill idle groups (one ray per group of K lanes) ---------- */
         bool started = false;
         for (;;) {
@@ -840,11 +899,15 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
                 continue;
             }
             const uint32_t avail = end - pos;
-            const uint32_t rk = (uint32_t)__popcll(mask & __lanemask_lt());
-            if (need && rk < avail) {
+            /* every lane of a needing group takes the group's ray: the
+             * group's rank among the needing leaders below it */
+            const uint32_t gb = lane - rank;
+            const bool gneed = ((mask >> gb) & 1ull) != 0;
+            const uint32_t rk = (uint32_t)__popcll(mask & ((1ull << gb) - 1ull));
+            if (gneed && rk < avail) {
                 const ViewDesc &v = views[uniform(cur_view)];
                 if (a.level == 0) {
-                    if (start_ray(v, pos + rk, r)) started = true;
+                    if (start_ray(v, pos + rk, r, rank == 0)) started = true;
                     else r.phase = PH_IDLE;
                 } else {
                     resume_ray(a.cont_in[v.cont_base + pos + rk], r);
@@ -854,13 +917,7 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
             const uint32_t taken = (uint32_t)__popcll(mask);
             pos += taken < avail ? taken : avail;
         }
-        const uint32_t gbase0 = lane - lane % K;
-        rays_started += (uint32_t)__popcll(__ballot(started));
-        if (__ballot(started) != 0) {
-            /* members mirror their group's first lane */
-            Ray c = shuffle_ray(r, (int)gbase0);
-            r = c;
-        }
+        rays_started += (uint32_t)__popcll(__ballot(started && lane % K == 0u));
         const uint64_t busy_mask = __ballot(r.phase != PH_IDLE);
         if (busy_mask == 0) {
             if (queue_empty && pend_tile == 0xFFFFFFFFu) break;
@@ -872,7 +929,7 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
             const uint64_t leaders = __ballot(r.phase != PH_IDLE && lane % K == 0u);
             const uint32_t nb = (uint32_t)__popcll(leaders);
             uint32_t K2 = 64u / nb;
-            if (K2 > 64u) K2 = 64u;
+            if (K2 * S > 64u) K2 = 64u / S;
             if (K2 > K) {
                 /* group i of the new layout takes the i-th busy ray */
                 uint32_t src = lane;
@@ -890,23 +947,63 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
             }
         }
         ++rounds;
-        /* ---- one lambda evaluation per lane ----------------------------- */
+        if (a.wave_log) {
+            const unsigned long long now = clock64();
+            c_refill += now - c_mark;
+            c_mark = now;
+        }
+        /* ---- S lambda evaluations per lane ------------------------------ */
         const ViewDesc &v = views[uniform(cur_view)];
         const uint32_t rank = lane % K;
         const uint32_t gbase = lane - rank;
+        const uint32_t KS = K * S;            /* samples per group and round */
         const bool busy = r.phase != PH_IDLE;
-        float lam = NAN;
-        F3 s = {0.f, 0.f, 0.f};
-        float ts = 0.f;
-        const bool valid = busy && sample_point(v, r, rank, K, s, ts);
-        if (valid) {
-            lam = eval_lambda<W, PREC>(v.e, s.x, s.y, s.z);
-            if (MARGIN && PREC != LYAP_PREC_FP64 && r.phase == PH_REFINE &&
-                v.refine == LYAP_REFINE_BISECT &&
-                !(__builtin_fabsf(lam - v.prm.opaqueThreshold) >= v.margin))
-                lam = eval_fp64_call<W>(v.e, s.x, s.y, s.z);
+        RoundStash &st = stash[threadIdx.x >> 6];
+        const uint64_t kmask = K >= 64u ? ~0ull : ((1ull << K) - 1ull);
+        uint64_t evm = 0;                     /* the group's stopping samples */
+        for (uint32_t sl = 0; sl < S; ++sl) {
+            const uint32_t q = sl * K + rank;
+            float lam = NAN;
+            F3 sp = {0.f, 0.f, 0.f};
+            float ts = 0.f;
+            const bool valid = busy && sample_point(v, r, q, KS, sp, ts);
+            if (valid) {
+                lam = eval_lambda<W, PREC>(v.e, sp.x, sp.y, sp.z);
+                if (MARGIN && PREC != LYAP_PREC_FP64 && r.phase == PH_REFINE &&
+                    v.refine == LYAP_REFINE_BISECT &&
+                    !(__builtin_fabsf(lam - v.prm.opaqueThreshold) >= v.margin))
+                    lam = eval_fp64_call<W>(v.e, sp.x, sp.y, sp.z);
+            }
+            /* does sample q stop the march (opaque, invalid, a near/far
+             * toggle, or jitter, whose step needs this lambda)? */
+            bool ev = true;
+            if (busy && r.phase == PH_MARCH && valid) {
+                const LyapParams &prm = v.prm;
+                const bool near_ = (r.flags & 1u) != 0;
+                const bool toggles = (r.index + q) > 0 && r.Ndt != r.Fdt &&
+                                     ((lam <= prm.nearThreshold) != near_);
+                ev = !(lam > prm.opaqueThreshold) || toggles || prm.jitter != 0.0f;
+            }
+            evm |= ((__ballot(ev) >> gbase) & kmask) << (sl * K);
+            /* lanes past the last whole group (after a regroup to a K that
+             * does not divide 64) hold no ray and have no stash slots */
+            if (gbase + K <= 64u) {
+                const uint32_t idx = gbase * S + q;
+                st.lam[idx] = lam;
+                st.t[idx] = ts;
+                st.x[idx] = sp.x;
+                st.y[idx] = sp.y;
+                st.z[idx] = sp.z;
+                st.ok[idx] = valid ? 1u : 0u;
+            }
         }
-        if (combine(v, r, lam, valid, ts, s, gbase, K, busy, rank == 0) && rank == 0)
+        wave_sync_lds();
+        if (a.wave_log) {
+            const unsigned long long now = clock64();
+            c_eval += now - c_mark;
+            c_mark = now;
+        }
+        if (combine(v, r, evm, st, gbase * S, KS, busy, rank == 0) && rank == 0)
             done_evals += r.evals;
         /* hand rays that used this launch's sample budget to the next launch */
         if (a.budget != 0) {
@@ -933,11 +1030,15 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
     if (lane == 0 && sum) atomicAdd(a.eval_counter, sum);
     if (a.wave_log && lane == 0) {
         const uint32_t wid = blockIdx.x * 4u + threadIdx.x / 64u;
-        unsigned long long *o = a.wave_log + 4ull * (wid + 8192ull * a.level);
+        unsigned long long *o = a.wave_log + 8ull * (wid + 8192ull * a.level);
         o[0] = rounds;
         o[1] = rays_started;
         o[2] = t_start;
         o[3] = clock64();
+        o[4] = c_eval;
+        o[5] = c_refill;
+        o[6] = c_combine;
+        o[7] = 0;
     }
 }
 
@@ -974,14 +1075,16 @@ static void fill_view(ViewDesc &v, const EvalDesc &e, const LyapCam &cam,
 
 /* ---- launch planning ----------------------------------------------------- */
 
-/* Lanes per ray while the queue still has rays: the power of two that lets
- * the launch's rays fill the resident lanes once, at least 4 (a group of 4
- * bisects two levels per trip and takes the six normal samples in two) and
- * at most 16. */
+/* Lanes per ray while the queue still has rays: one when the launch has
+ * rays for at least four generations of the resident lanes (every sample a
+ * lane evaluates is then one the ray needs, and the control work is not
+ * replicated), otherwise the power of two that gives about four
+ * generations, at most 16 (a small frame's rays finish sooner with several
+ * lanes speculating ahead). */
 static uint32_t default_lanes_per_ray(uint32_t resident_lanes, uint64_t rays)
 {
-    uint32_t k = 4;
-    while (k < 16u && (uint64_t)resident_lanes >= rays * (uint64_t)(2 * k)) k *= 2;
+    uint32_t k = 1;
+    while (k < 16u && rays * (uint64_t)k < 4ull * resident_lanes) k *= 2;
     return k;
 }
 
@@ -989,9 +1092,14 @@ static uint32_t default_lanes_per_ray(uint32_t resident_lanes, uint64_t rays)
  * per work-queue claim. A claim is one generation of a wave's groups, so
  * claims are many times the number of waves and the queue balances. */
 struct LaunchPlan {
-    uint32_t blocks, lanes_per_ray, tile_rays, resident_lanes;
+    uint32_t blocks, lanes_per_ray, tile_rays, resident_lanes, slots;
     bool direct;
 };
+
+/* Samples each lane evaluates per round unless tuned. More than one halves
+ * the lanes a ray's control work is replicated on, but measured on the
+ * 512x512 frame one slot with four lanes per ray is fastest. */
+static const uint32_t kDefaultSlots = 1u;
 
 /* March-sample budgets of the launch levels; the last level has none. */
 static const uint32_t kLevelBudget[4] = {32u, 64u, 160u, 0u};
@@ -1037,14 +1145,23 @@ static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, const LyapTuni
     if (tune.waves_per_cu && (tune.waves_per_cu + 3u) / 4u < per_cu) per_cu = (tune.waves_per_cu + 3u) / 4u;
     p.blocks = cus * per_cu;
     p.resident_lanes = p.blocks * 256u;
-    p.lanes_per_ray = tune.lanes_per_ray ? tune.lanes_per_ray
-                                         : default_lanes_per_ray(p.resident_lanes, rays);
+    p.slots = tune.slots ? tune.slots : kDefaultSlots;
+    if (p.slots > kMaxSlots) p.slots = kMaxSlots;
+    if (tune.lanes_per_ray) {
+        p.lanes_per_ray = tune.lanes_per_ray;
+    } else {
+        /* the default samples per ray and round, spread over the slots */
+        const uint32_t ks = default_lanes_per_ray(p.resident_lanes, rays);
+        p.lanes_per_ray = ks / p.slots > 0u ? ks / p.slots : 1u;
+    }
     if (p.lanes_per_ray > 64u) p.lanes_per_ray = 64u;
     if (p.lanes_per_ray == 0) p.lanes_per_ray = 1;
+    if (p.lanes_per_ray * p.slots > 64u) p.lanes_per_ray = 64u / p.slots;
     p.tile_rays = tune.tile_rays ? tune.tile_rays : 64u / p.lanes_per_ray;
     if (p.tile_rays == 0) p.tile_rays = 1;
     if (direct) {
         p.lanes_per_ray = 1;
+        p.slots = 1;
         p.tile_rays = 64;
     }
     return p;
@@ -1173,6 +1290,7 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         a.total_tiles = total_tiles;
         a.tile_rays = plan.tile_rays;
         a.lanes_per_ray = plan.lanes_per_ray;
+        a.slots = plan.slots;
         a.tile_counter = res.counters + 4 + level;
         a.eval_counter = (unsigned long long *)(res.counters + 2);
         a.wave_log = (unsigned long long *)ctx->debug_wave_log;
@@ -1313,6 +1431,7 @@ static LyapTuning merged_tuning(const lyap_ctx *ctx, const LyapTuning *t)
         if (t->tile_rays) m.tile_rays = t->tile_rays;
         if (t->waves_per_cu) m.waves_per_cu = t->waves_per_cu;
         if (t->lanes_per_ray) m.lanes_per_ray = t->lanes_per_ray;
+        if (t->slots) m.slots = t->slots;
     }
     return m;
 }
@@ -1455,14 +1574,19 @@ extern "C" int lyap_autotune(lyap_ctx *ctx, const LyapView *views, unsigned n_vi
     hipEvent_t e0, e1;
     LYAP_HIP(ctx, hipEventCreate(&e0));
     LYAP_HIP(ctx, hipEventCreate(&e1));
-    /* candidates: lanes per ray x tile length in generations */
-    const uint32_t lanes[] = {2u, 4u, 8u, 16u};
-    const uint32_t gens[] = {1u, 2u, 4u};
+    /* candidates: samples per lane x lanes per ray x tile length in
+     * generations of the wave's groups */
+    const uint32_t slots[] = {1u, 2u, 4u};
+    const uint32_t lanes[] = {1u, 2u, 4u, 8u, 16u};
+    const uint32_t gens[] = {1u, 2u};
     float best_ms = 1e30f;
     LyapTuning best_t{};
+    for (uint32_t sl : slots) {
     for (uint32_t k : lanes) {
+        if (k * sl < 2u || k * sl > 32u) continue;
         for (uint32_t g : gens) {
             LyapTuning t{};
+            t.slots = sl;
             t.lanes_per_ray = k;
             t.tile_rays = g * (64u / k);
             std::vector<ViewDesc> trial = vd;
@@ -1480,6 +1604,8 @@ extern "C" int lyap_autotune(lyap_ctx *ctx, const LyapView *views, unsigned n_vi
             LYAP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
             if (ms < best_ms) { best_ms = ms; best_t = t; }
         }
+        if (rc) break;
+    }
         if (rc) break;
     }
     (void)hipEventDestroy(e0);
