@@ -286,6 +286,86 @@ int lyap_frame_replay(lyap_ctx *ctx, lyap_frame *frame);
 void lyap_frame_destroy(lyap_frame *frame);
 
 /* ------------------------------------------------------------------------ */
+/* Headless parameter editing (the viewer's keyboard and spaceball)          */
+/* ------------------------------------------------------------------------ */
+
+/* Control modes of lyap_interactive.cu:43-55. */
+enum {
+    LYAP_EDIT_LOCK = 0, LYAP_EDIT_ROTPAN, LYAP_EDIT_MAG, LYAP_EDIT_NTHRESH,
+    LYAP_EDIT_OTHRESH, LYAP_EDIT_CTHRESH, LYAP_EDIT_D, LYAP_EDIT_JITTER,
+    LYAP_EDIT_L_PAN, LYAP_EDIT_L_RANGE, LYAP_EDIT_L_COLOR
+};
+/* What an edit invalidated (update_cam/lights/calc/render,
+ * lyap_interactive.cu:57-60), and the quit key. */
+enum {
+    LYAP_UPDATE_CAM = 1, LYAP_UPDATE_LIGHTS = 2, LYAP_UPDATE_CALC = 4,
+    LYAP_UPDATE_RENDER = 8, LYAP_UPDATE_QUIT = 16
+};
+
+/* The viewer's editable state (lyap_interactive.cu:36-71, params.cu:9). */
+typedef struct LyapEditor {
+    LyapParams prm;
+    LyapCam cam;
+    LyapLight lights[LYAP_MAX_LIGHTS];
+    LyapLight home_light;        /* L0: the first light as initialised */
+    uint32_t num_lights;
+    int32_t mode;                /* LYAP_EDIT_* */
+    int32_t cur_light;           /* -1: the camera is current (curC), else a light */
+    int32_t light_index;         /* curL */
+    int32_t dominant;            /* spaceball axis locking */
+    uint32_t window_w, window_h, render_denominator;
+    uint32_t pending;            /* LYAP_UPDATE_* since the last commit */
+} LyapEditor;
+
+/* params_init plus the viewer's initial control state. */
+int lyap_editor_init(LyapEditor *ed, unsigned window_w, unsigned window_h);
+/* glut_keyboard (lyap_interactive.cu:311-463); returns LYAP_UPDATE_* flags. */
+unsigned lyap_editor_key(LyapEditor *ed, int key);
+/* glut_spaceball_rot / _pan / _but (lyap_interactive.cu:144-309). */
+unsigned lyap_editor_spaceball_rot(LyapEditor *ed, int x, int y, int z);
+unsigned lyap_editor_spaceball_pan(LyapEditor *ed, int x, int y, int z);
+unsigned lyap_editor_spaceball_button(LyapEditor *ed, int button);
+/* update_scene (lyap_interactive.cu:118-137): recalculates the lights and
+ * the camera for an image_w x image_h frame as the pending flags ask; returns
+ * and clears them. A caller then renders (or submits) the new frame. */
+unsigned lyap_editor_commit(LyapEditor *ed, unsigned image_w, unsigned image_h);
+
+/* ------------------------------------------------------------------------ */
+/* Scripted fly-through (scale.pl)                                           */
+/* ------------------------------------------------------------------------ */
+
+/* easeInOutQuart(t, b, c, d) (scale.pl:5-11). */
+double lyap_ease_in_out_quart(double t, double b, double c, double d);
+/* The frame times of `for ($t = 0; $t <= 1.00001; $t += step)`
+ * (scale.pl:15): their number, and up to `cap` of them in t_out. */
+unsigned lyap_flythrough_frames(double step, double *t_out, unsigned cap);
+/* The camera of scale.pl:33-50 for an eased value i: the view axis (4,4,4)
+ * turned by nlerp(-20, +20 degrees about `up`, i), the camera one unit
+ * behind (4 - 0.9 i) along each axis. Sets C and Q; the caller then calls
+ * lyap_scene_cam_recalculate. */
+void lyap_flythrough_camera(double eased, LyapCam *cam);
+
+/* ------------------------------------------------------------------------ */
+/* On-disk formats                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* exps.raw: raw FP32, x fastest (lyap_calculate.cu:79-86). Host buffers. */
+int lyap_write_volume_raw(const char *path, const float *exps, size_t n);
+/* Points_*.raw: raw LyapPoint[w*h] (lyap_interactive.cu:616-655). */
+int lyap_write_points_raw(const char *path, const LyapPoint *points, size_t n);
+/* Render_*.ppm: "P3\n<w> <h>\n255\n" then "%3d %3d %3d " per pixel in
+ * buffer order and a newline (lyap_interactive.cu:569-613). */
+int lyap_write_ppm(const char *path, const LyapRGBA *rgba, unsigned w, unsigned h);
+/* The parameter-encoding leaf name "<kind>_<stamp>_<w>x<h>_<seq>_cx=..._ot=..."
+ * (lyap_interactive.cu:577-591), kind "Render" or "Points". */
+int lyap_output_leaf(char *out, size_t cap, const char *kind, long stamp, unsigned w,
+                     unsigned h, const char *sequence, const LyapCam *cam,
+                     const LyapParams *prm);
+/* "<leaf>_time=<h>h<mm>m<ss>s.<ext>" (lyap_interactive.cu:594, 607). */
+int lyap_output_name(char *out, size_t cap, const char *leaf, unsigned hours,
+                     unsigned minutes, unsigned seconds, const char *ext);
+
+/* ------------------------------------------------------------------------ */
 /* Serving: continuous batching of render and volume requests               */
 /* ------------------------------------------------------------------------ */
 

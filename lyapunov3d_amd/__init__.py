@@ -51,6 +51,10 @@ REQ_VOLUME = 1
 SCHEDULE_BATCHED = 0
 SCHEDULE_DIRECT = 1
 
+(EDIT_LOCK, EDIT_ROTPAN, EDIT_MAG, EDIT_NTHRESH, EDIT_OTHRESH, EDIT_CTHRESH, EDIT_D, EDIT_JITTER,
+ EDIT_L_PAN, EDIT_L_RANGE, EDIT_L_COLOR) = range(11)
+UPDATE_CAM, UPDATE_LIGHTS, UPDATE_CALC, UPDATE_RENDER, UPDATE_QUIT = 1, 2, 4, 8, 16
+
 
 # ---------------------------------------------------------------- types ----
 
@@ -180,6 +184,18 @@ class ServerStats(C.Structure):
     ]
 
 
+class Editor(C.Structure):
+    """LyapEditor: the viewer's editable state (lyap_interactive.cu:36-71)."""
+    _fields_ = [
+        ("prm", LyapParams), ("_pad0", C.c_uint32 * 2), ("cam", LyapCam),
+        ("lights", LyapLight * MAX_LIGHTS), ("home_light", LyapLight),
+        ("num_lights", C.c_uint32), ("mode", C.c_int32), ("cur_light", C.c_int32),
+        ("light_index", C.c_int32), ("dominant", C.c_int32), ("window_w", C.c_uint32),
+        ("window_h", C.c_uint32), ("render_denominator", C.c_uint32), ("pending", C.c_uint32),
+        ("_pad1", C.c_uint32 * 3),
+    ]
+
+
 assert C.sizeof(LyapParams) == 56
 assert C.sizeof(LyapPoint) == 36
 assert C.sizeof(LyapCam) == 224
@@ -244,6 +260,22 @@ _SIGS = {
     "lyap_server_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "lyap_server_drain": (C.c_int, [_P, C.c_double]),
     "lyap_server_finished": (C.c_int, [_P, C.POINTER(C.c_uint64), C.c_uint, C.POINTER(C.c_uint)]),
+    "lyap_editor_init": (C.c_int, [C.POINTER(Editor), C.c_uint, C.c_uint]),
+    "lyap_editor_key": (C.c_uint, [C.POINTER(Editor), C.c_int]),
+    "lyap_editor_spaceball_rot": (C.c_uint, [C.POINTER(Editor), C.c_int, C.c_int, C.c_int]),
+    "lyap_editor_spaceball_pan": (C.c_uint, [C.POINTER(Editor), C.c_int, C.c_int, C.c_int]),
+    "lyap_editor_spaceball_button": (C.c_uint, [C.POINTER(Editor), C.c_int]),
+    "lyap_editor_commit": (C.c_uint, [C.POINTER(Editor), C.c_uint, C.c_uint]),
+    "lyap_ease_in_out_quart": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double]),
+    "lyap_flythrough_frames": (C.c_uint, [C.c_double, C.POINTER(C.c_double), C.c_uint]),
+    "lyap_flythrough_camera": (None, [C.c_double, C.POINTER(LyapCam)]),
+    "lyap_write_volume_raw": (C.c_int, [C.c_char_p, _P, C.c_size_t]),
+    "lyap_write_points_raw": (C.c_int, [C.c_char_p, _P, C.c_size_t]),
+    "lyap_write_ppm": (C.c_int, [C.c_char_p, _P, C.c_uint, C.c_uint]),
+    "lyap_output_leaf": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.c_long, C.c_uint, C.c_uint,
+                                   C.c_char_p, C.POINTER(LyapCam), C.POINTER(LyapParams)]),
+    "lyap_output_name": (C.c_int, [C.c_char_p, C.c_size_t, C.c_char_p, C.c_uint, C.c_uint, C.c_uint,
+                                   C.c_char_p]),
 }
 
 
@@ -637,3 +669,75 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------- editing, files ----
+
+def editor(window_w: int = 512, window_h: int = 512) -> Editor:
+    """A LyapEditor initialised like the viewer (params_init + its controls)."""
+    ed = Editor()
+    _check(load().lyap_editor_init(C.byref(ed), window_w, window_h), None, "lyap_editor_init")
+    return ed
+
+
+def ease_in_out_quart(t: float, b: float = 0.0, c: float = 1.0, d: float = 1.0) -> float:
+    return float(load().lyap_ease_in_out_quart(t, b, c, d))
+
+
+def flythrough_times(step: float = 0.01) -> list:
+    """The frame times t of scale.pl's loop (accumulated in double)."""
+    n = load().lyap_flythrough_frames(step, None, 0)
+    buf = (C.c_double * max(n, 1))()
+    load().lyap_flythrough_frames(step, buf, n)
+    return [buf[i] for i in range(n)]
+
+
+def flythrough_camera(eased: float, w: int, h: int, M: Optional[float] = None) -> LyapCam:
+    """The scripted camera for an eased value, recalculated for a w x h frame
+    (M from params_init unless given)."""
+    _, cam, _, _ = params_init()
+    if M is not None:
+        cam.M = M
+    load().lyap_flythrough_camera(eased, C.byref(cam))
+    load().lyap_scene_cam_recalculate(C.byref(cam), w, h, 1)
+    return cam
+
+
+def output_leaf(kind: str, stamp: int, w: int, h: int, sequence: str, cam: LyapCam, prm: LyapParams) -> str:
+    buf = C.create_string_buffer(512)
+    _check(load().lyap_output_leaf(buf, 512, kind.encode(), stamp, w, h, sequence.encode(), C.byref(cam),
+                                   C.byref(prm)), None, "lyap_output_leaf")
+    return buf.value.decode()
+
+
+def output_name(leaf: str, hours: int, minutes: int, seconds: int, ext: str) -> str:
+    buf = C.create_string_buffer(600)
+    _check(load().lyap_output_name(buf, 600, leaf.encode(), hours, minutes, seconds, ext.encode()), None,
+           "lyap_output_name")
+    return buf.value.decode()
+
+
+def write_ppm(path: str, rgba, w: int, h: int):
+    """rgba: a host buffer of w*h RGBA bytes (numpy uint8 array or bytes)."""
+    import numpy as np
+    a = np.ascontiguousarray(np.asarray(rgba, dtype=np.uint8).reshape(-1))
+    if a.size < 4 * w * h:
+        raise LyapError(ERR_INVALID_ARGUMENT, "rgba buffer smaller than w*h*4")
+    _check(load().lyap_write_ppm(path.encode(), a.ctypes.data_as(C.c_void_p), w, h), None, "lyap_write_ppm")
+
+
+def write_volume_raw(path: str, exps):
+    import numpy as np
+    a = np.ascontiguousarray(np.asarray(exps, dtype=np.float32).reshape(-1))
+    _check(load().lyap_write_volume_raw(path.encode(), a.ctypes.data_as(C.c_void_p), a.size), None,
+           "lyap_write_volume_raw")
+
+
+def write_points_raw(path: str, points, n: int):
+    """points: a host buffer of n LyapPoint records (36 bytes each)."""
+    import numpy as np
+    a = np.ascontiguousarray(np.asarray(points).view(np.uint8).reshape(-1))
+    if a.size < 36 * n:
+        raise LyapError(ERR_INVALID_ARGUMENT, "points buffer smaller than n records")
+    _check(load().lyap_write_points_raw(path.encode(), a.ctypes.data_as(C.c_void_p), n), None,
+           "lyap_write_points_raw")

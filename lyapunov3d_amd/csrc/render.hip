@@ -846,6 +846,9 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
         if (total == 0) return;
         K0 = 1;
         while (K0 < 64u && (uint64_t)a.resident_lanes >= (uint64_t)total * (2u * K0)) K0 *= 2;
+        /* one more doubling while the rays still fill two thirds of the
+         * resident lanes: handed-over rays are long, so shorter trips pay */
+        if (K0 < 64u && 2ull * a.resident_lanes >= 3ull * total * K0) K0 *= 2;
         K0 = K0 / S > 0u ? K0 / S : 1u;
         tile = uniform(64u / K0);
     }
@@ -1102,7 +1105,7 @@ struct LaunchPlan {
 static const uint32_t kDefaultSlots = 1u;
 
 /* March-sample budgets of the launch levels; the last level has none. */
-static const uint32_t kLevelBudget[4] = {32u, 64u, 160u, 0u};
+static const uint32_t kLevelBudget[4] = {24u, 64u, 192u, 0u};
 
 template <int W, int PREC, bool MARGIN>
 static uint32_t resident_blocks_per_cu()

@@ -57,11 +57,14 @@ LAYOUT_C = r"""
 int main(void) {
   S(LyapParams) S(LyapPoint) S(LyapCam) S(LyapRGBA) S(LyapView) S(LyapRequest)
   S(LyapVolumeMode) S(LyapRenderMode) S(LyapTuning) S(LyapServerConfig) S(LyapResult) S(LyapServerStats)
+  S(LyapEditor)
   O(LyapCam, Q) O(LyapCam, ambient) O(LyapCam, chaosColor) O(LyapView, prm) O(LyapView, seq)
   O(LyapView, rgba) O(LyapView, num_lights) O(LyapRequest, view) O(LyapRequest, vol_prm)
   O(LyapRequest, vol_out) O(LyapRequest, deadline_ms) O(LyapServerConfig, mode) O(LyapServerConfig, vol_mode)
   O(LyapResult, submit_ms) O(LyapResult, lambda) O(LyapServerConfig, slice_ms)
   O(LyapServerStats, resident_bytes) O(LyapServerStats, device_ms)
+  O(LyapEditor, cam) O(LyapEditor, lights) O(LyapEditor, home_light) O(LyapEditor, num_lights)
+  O(LyapEditor, pending)
   return 0;
 }
 """
@@ -83,7 +86,7 @@ def test_struct_layouts_match_c(tmp_path):
           "LyapRGBA": lyap.RGBA, "LyapView": lyap.View, "LyapRequest": lyap.Request,
           "LyapVolumeMode": lyap.VolumeMode, "LyapRenderMode": lyap.RenderMode, "LyapTuning": lyap.Tuning,
           "LyapServerConfig": lyap.ServerConfig, "LyapResult": lyap.Result,
-          "LyapServerStats": lyap.ServerStats}
+          "LyapServerStats": lyap.ServerStats, "LyapEditor": lyap.Editor}
     for name, cls in py.items():
         assert C.sizeof(cls) == c[name][0], name
     assert c["LyapParams"][0] == 56 and c["LyapPoint"][0] == 36 and c["LyapCam"][0] == 224
@@ -96,6 +99,8 @@ def test_struct_layouts_match_c(tmp_path):
               "LyapServerConfig.vol_mode": lyap.ServerConfig.vol_mode, "LyapResult.submit_ms": lyap.Result.submit_ms,
               "LyapResult.lambda": lyap.Result.lambda_, "LyapServerConfig.slice_ms": lyap.ServerConfig.slice_ms,
               "LyapServerStats.resident_bytes": lyap.ServerStats.resident_bytes,
-              "LyapServerStats.device_ms": lyap.ServerStats.device_ms}
+              "LyapServerStats.device_ms": lyap.ServerStats.device_ms, "LyapEditor.cam": lyap.Editor.cam,
+              "LyapEditor.lights": lyap.Editor.lights, "LyapEditor.home_light": lyap.Editor.home_light,
+              "LyapEditor.num_lights": lyap.Editor.num_lights, "LyapEditor.pending": lyap.Editor.pending}
     for k, f in fields.items():
         assert f.offset == c[k][0], k
