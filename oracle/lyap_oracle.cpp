@@ -37,7 +37,10 @@
  *
  * Pinning: tests/test_oracle.py checks these functions against exact rational
  * hand computations, closed forms on the diagonal, permutation invariants,
- * an mpmath brute force, and the analytic one-letter raymarch scene.
+ * an mpmath brute force, the analytic one-letter raymarch scene, and the
+ * closed step recurrence of a constant-lambda ray (jitter, near step).
+ * Parity unpinned: the order of the a/c sums of transparent samples
+ * (kernel.cu:370-375) is restated from reading only.
  */
 #include <cmath>
 #include <cstdint>

@@ -252,3 +252,67 @@ def test_shade_closed_form():
     pt[6] = np.nan                           # miss -> transparent black
     col, rgba = O.shade(pt, cam, lights, 1)
     assert np.all(col == 0) and np.all(rgba == 0)
+
+
+@pytest.mark.parametrize("x0,lam_mode", [(2.8, O.F64), (2.8, O.F32W), (3.9, O.F64), (3.9, O.F32W)])
+def test_jitter_and_near_step_on_a_constant_lambda_ray(x0, lam_mode):
+    """A ray along +z at fixed x sees one lambda (sequence 'A' reads only x),
+    so the reference march (kernel.cu:318-384) reduces to a closed
+    recurrence: every step is dt * jit with jit = 1 -/+ frac(lambda) * jitter
+    (the sign branch of :339-342), dt = Fdt for the first step and Ndt =
+    Fdt / nearMultiplier after the first sample at or below nearThreshold
+    (:377-380). The ray is transparent and leaves the box; its evaluation
+    count is the number of samples the recurrence takes, computed here in
+    float32. x0 = 2.8 gives lambda = ln 0.8 < 0 (toggles to the near step),
+    x0 = 3.9 a chaotic lambda > 0 (no toggle)."""
+    w = h = 16
+    prm, cam, lights, n = lyap.params_init()
+    prm.settle, prm.accum = 100, 200
+    prm.lMin, prm.lMax = 2.0, 4.0
+    prm.opaqueThreshold = -0.75
+    prm.chaosThreshold = -0.5
+    prm.nearThreshold = 0.0
+    prm.nearMultiplier = 2.0
+    prm.jitter = 0.5
+    prm.stepMethod = 1
+    prm.depth = 40.0
+    cam.Q.x = cam.Q.y = cam.Q.z = 0.0
+    cam.Q.w = 1.0
+    cam.C.x, cam.C.y, cam.C.z = x0, 3.0, 1.5
+    cam.M = 0.5
+    lyap.cam_recalculate(cam, w, h, 1)
+    lyap.lights_recalculate(lights, n)
+    mode = lyap.render_mode(refine=lyap.REFINE_WALK)
+    ok, V, t0, t1 = O.ray_setup(cam, prm, 8, 8)
+    assert ok and V[0] == 0.0 and V[1] == 0.0           # the centre ray keeps x
+    lam = np.float32(O.lyap_points([[x0, 3.0, 2.0]], prm.d, prm.settle, prm.accum, "A", mode=lam_mode)[0])
+    assert lam > prm.opaqueThreshold and (lam < 0) == (x0 < 3.0)
+    f = np.float32
+    frac = f(lam - np.trunc(lam))
+    jit = f(1.0 - float(f(frac * f(prm.jitter)))) if frac < 0 else f(1.0 + float(f(frac * f(prm.jitter))))
+    Fdt = f(f(f(t1) - f(t0)) / f(prm.depth))
+    Ndt = f(Fdt / f(prm.nearMultiplier))
+    t, dt, near, count = f(t0), Fdt, False, 1
+    while True:
+        t = f(t + f(dt * jit))
+        if t > f(t1):
+            break
+        count += 1
+        if lam <= prm.nearThreshold and not near:
+            near, dt = True, Ndt
+    out = O.render(cam, prm, "A", lights, n, w, h, mode=mode, lam=lam_mode, rows=(8, 100))
+    assert np.isnan(out["depth"][8, 8])                 # a miss
+    assert out["evals"][8, 8] == count
+    # without the near step and without jitter the counts are what the
+    # plain uniform march gives
+    prm.nearMultiplier = 1.0
+    prm.jitter = 0.0
+    out = O.render(cam, prm, "A", lights, n, w, h, mode=mode, lam=lam_mode, rows=(8, 100))
+    t, c2 = f(t0), 1
+    while True:
+        t = f(t + Fdt)
+        if t > f(t1):
+            break
+        c2 += 1
+    assert out["evals"][8, 8] == c2
+    assert count < c2 if lam > 0 else count > c2
