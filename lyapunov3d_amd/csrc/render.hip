@@ -1105,7 +1105,17 @@ struct LaunchPlan {
 static const uint32_t kDefaultSlots = 1u;
 
 /* March-sample budgets of the launch levels; the last level has none. */
-static const uint32_t kLevelBudget[4] = {24u, 64u, 192u, 0u};
+static const uint32_t kLevelBudget[4] = {32u, 64u, 192u, 0u};
+
+/* Level 0 hands rays over sooner when several lanes march each ray (a small
+ * launch, where the long rays decide the frame time); with one lane per ray
+ * the launch is large and handing over costs more than it saves. */
+static uint32_t level_budget(uint32_t level, const LaunchPlan &plan)
+{
+    if (plan.direct) return 0u;
+    if (level == 0 && plan.lanes_per_ray >= 4u) return 24u;
+    return kLevelBudget[level];
+}
 
 template <int W, int PREC, bool MARGIN>
 static uint32_t resident_blocks_per_cu()
@@ -1298,7 +1308,7 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         a.eval_counter = (unsigned long long *)(res.counters + 2);
         a.wave_log = (unsigned long long *)ctx->debug_wave_log;
         a.level = level;
-        a.budget = plan.direct ? 0u : kLevelBudget[level];
+        a.budget = level_budget(level, plan);
         a.direct = plan.direct ? 1u : 0u;
         a.resident_lanes = plan.resident_lanes;
         a.cont_in = in_buf[level];
