@@ -166,6 +166,14 @@ int lyap_calc_points(lyap_ctx *ctx, float *d_out, const float *d_xyz, size_t n,
                      const LyapParams *prm, const int32_t *seq,
                      const LyapVolumeMode *mode);
 
+/* The PRODUCT form's lambda and the lambdas of the two halves of its
+ * accumulation window for each point (a convergence diagnostic: a point
+ * whose halves disagree has not settled on its attractor). No reference
+ * counterpart. */
+int lyap_calc_points_halves(lyap_ctx *ctx, float *d_lambda, float *d_half1, float *d_half2,
+                            const float *d_xyz, size_t n, const LyapParams *prm,
+                            const int32_t *seq, unsigned renorm);
+
 /* Statistics of the last volume/points call on this context: voxels whose
  * lambda was re-evaluated in FP64. Waits for that call to finish. */
 int lyap_last_reevaluated(lyap_ctx *ctx, uint64_t *count);

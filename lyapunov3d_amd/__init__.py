@@ -239,6 +239,8 @@ _SIGS = {
     "lyap_calc_volume": (C.c_int, [_P, _P, C.POINTER(LyapParams), C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint, C.c_uint, C.c_uint, C.POINTER(VolumeMode)]),
     "lyap_calc_points": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(LyapParams), C.POINTER(C.c_int32), C.POINTER(VolumeMode)]),
     "lyap_last_reevaluated": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "lyap_calc_points_halves": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.POINTER(LyapParams),
+                                          C.POINTER(C.c_int32), C.c_uint]),
     "lyap_calc_render": (C.c_int, [_P, _P, _P, C.POINTER(LyapCam), C.POINTER(LyapParams), C.POINTER(C.c_int32), _P, C.c_uint, C.c_uint, C.c_uint, _P, _P, _P, C.POINTER(RenderMode)]),
     "lyap_last_evals": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "lyap_debug_wave_log": (C.c_int, [_P, _P]),
@@ -472,6 +474,12 @@ class Context:
         rc = self.lib.lyap_calc_points(self.h, out_ptr, xyz_ptr, n, C.byref(prm), seq_array(seq),
                                        C.byref(mode) if mode is not None else None)
         _check(rc, self.h, "lyap_calc_points")
+
+    def calc_points_halves(self, lam_ptr: int, h1_ptr: int, h2_ptr: int, xyz_ptr: int, n: int,
+                           prm: LyapParams, seq, renorm: int = 16):
+        rc = self.lib.lyap_calc_points_halves(self.h, lam_ptr, h1_ptr, h2_ptr, xyz_ptr, n, C.byref(prm),
+                                              seq_array(seq), renorm)
+        _check(rc, self.h, "lyap_calc_points_halves")
 
     def last_reevaluated(self) -> int:
         c = C.c_uint64()

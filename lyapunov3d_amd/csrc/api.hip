@@ -244,6 +244,20 @@ int lyap_calc_points(lyap_ctx *ctx, float *d_out, const float *d_xyz, size_t n,
     return launch_volume(ctx, d_out, e, zero, zero, 0, 0, 0, d_xyz, n, mode);
 }
 
+int lyap_calc_points_halves(lyap_ctx *ctx, float *d_lambda, float *d_half1, float *d_half2,
+                            const float *d_xyz, size_t n, const LyapParams *prm,
+                            const int32_t *seq, unsigned renorm)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    if (n == 0) return LYAP_OK;
+    if (!d_lambda || !d_half1 || !d_half2 || !d_xyz)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null buffer");
+    EvalDesc e;
+    int rc = check_params(ctx, prm, seq, &e, renorm);
+    if (rc) return rc;
+    return launch_points_halves(ctx, d_lambda, d_half1, d_half2, e, d_xyz, n);
+}
+
 int lyap_last_reevaluated(lyap_ctx *ctx, uint64_t *count)
 {
     if (!ctx || !count) return LYAP_ERR_INVALID_ARGUMENT;

@@ -70,6 +70,9 @@ int launch_volume_slab(lyap_ctx *ctx, float *out, const EvalDesc &e,
                        unsigned ny, unsigned nz, unsigned z_begin, unsigned z_end,
                        const float *xyz, size_t npoints, const LyapVolumeMode &mode);
 
+int launch_points_halves(lyap_ctx *ctx, float *lam, float *h1, float *h2, const EvalDesc &e,
+                         const float *xyz, size_t n);
+
 /* Server launch slots (render.hip). */
 struct RenderSlot;
 RenderSlot *render_slot_create();

@@ -110,6 +110,31 @@ __global__ __launch_bounds__(256) void volume_fp64(VolumeArgs a)
     }
 }
 
+/* Product form with the two window halves, per point. */
+template <int L>
+__global__ __launch_bounds__(256) void points_halves(VolumeArgs a, float *h1, float *h2)
+{
+    const uint64_t idx = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (idx >= a.n) return;
+    float x, y, z;
+    voxel_coords(a, idx, x, y, z);
+    const ProductResult r = lambda_product<L, true>(x, y, z, a.e);
+    a.out[idx] = r.lam;
+    h1[idx] = r.half1;
+    h2[idx] = r.half2;
+}
+
+template <int L>
+struct HalvesLaunch {
+    static int run(lyap_ctx *ctx, VolumeArgs &a, float *h1, float *h2)
+    {
+        const uint64_t blocks = (a.n + 255) / 256;
+        hipLaunchKernelGGL(points_halves<L>, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a, h1, h2);
+        LYAP_HIP(ctx, hipGetLastError());
+        return LYAP_OK;
+    }
+};
+
 __global__ void zero_u32(uint32_t *p, uint32_t n)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -180,6 +205,23 @@ int launch_volume(lyap_ctx *ctx, float *out, const EvalDesc &e,
                   const LyapVolumeMode &mode)
 {
     return launch_volume_slab(ctx, out, e, lo, hi, nx, ny, nz, 0, nz, xyz, npoints, mode);
+}
+
+int launch_points_halves(lyap_ctx *ctx, float *lam, float *h1, float *h2, const EvalDesc &e,
+                         const float *xyz, size_t n)
+{
+    if (n == 0) return LYAP_OK;
+    if (n > 0xFFFFFFFFull) return fail(ctx, LYAP_ERR_UNSUPPORTED, "more than 2^32-1 points");
+    VolumeArgs a{};
+    a.out = lam;
+    a.xyz = xyz;
+    a.n = n;
+    a.e = e;
+    switch (e.L) {
+        LYAP_L_CASES(HalvesLaunch, ctx, a, h1, h2)
+    default:
+        return fail(ctx, LYAP_ERR_BAD_SEQUENCE, "sequence period %d outside 1..%d", e.L, LYAP_MAX_SEQ);
+    }
 }
 
 /* z slices [z_begin, z_end) of the grid; out points at slice z_begin. */
