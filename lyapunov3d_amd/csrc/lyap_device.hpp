@@ -585,6 +585,80 @@ struct ProductFull {
     }
 };
 
+/* Calls f.template go<N>() for n in [1, W); nothing for n = 0. Partial
+ * blocks are specialised by their start or length this way, so none of
+ * their steps carries a per-step guard. n is wave-uniform. */
+template <int W, int N, class F>
+__device__ __forceinline__ void with_len(uint32_t n, F &f)
+{
+    if constexpr (N >= 1) {
+        if (n == (uint32_t)N) f.template go<N>();
+        else with_len<W, N - 1>(n, f);
+    }
+}
+
+/* Window positions [LO, HI) of the product form; positions past W/2 are
+ * guarded by j < hi when GUARD (they may lie past a block of B < W). */
+template <int W, int LO, int HI, bool GUARD>
+__device__ __forceinline__ void product_range(float &u, ProductState &st,
+                                              const float (&b)[W], uint32_t hi)
+{
+    const float u0 = u;
+    const ProductState s0 = st;
+#pragma unroll
+    for (int j = LO; j < HI; ++j) {
+        if (!GUARD || j <= W / 2 || (uint32_t)j < hi) {
+            u = umap(u, b[j]);
+            st.M *= __builtin_fabsf(u);
+        }
+    }
+    if (st.M < kUnderflowGuard) {
+        float w = u0;
+        st = s0;
+#pragma unroll
+        for (int j = LO; j < HI; ++j) {
+            if (!GUARD || j <= W / 2 || (uint32_t)j < hi) {
+                w = umap(w, b[j]);
+                st.M *= __builtin_fabsf(w);
+                renorm(st);
+            }
+        }
+    }
+    renorm(st);
+}
+
+/* Settle steps [0, N) / product steps [0, N) / the rest of a block from N. */
+template <int W>
+struct SettlePrefix {
+    float &u;
+    const float (&b)[W];
+    template <int N>
+    __device__ __forceinline__ void go()
+    {
+#pragma unroll
+        for (int j = 0; j < N; ++j) u = umap(u, b[j]);
+    }
+};
+
+template <int W>
+struct ProductPrefix {
+    float &u;
+    ProductState &st;
+    const float (&b)[W];
+    template <int N>
+    __device__ __forceinline__ void go() { product_range<W, 0, N, false>(u, st, b, 0); }
+};
+
+template <int W>
+struct ProductHead {
+    float &u;
+    ProductState &st;
+    const float (&b)[W];
+    uint32_t hi;
+    template <int N>
+    __device__ __forceinline__ void go() { product_range<W, N, W, true>(u, st, b, hi); }
+};
+
 /* FP32 product form, any period L <= W. */
 template <int W>
 __device__ __forceinline__ float lambda_product_w(float x, float y, float z,
@@ -597,15 +671,27 @@ __device__ __forceinline__ float lambda_product_w(float x, float y, float z,
         SettleFull<W> f{u, b, e.set_blocks};
         with_block_len<W, W>(e.B, f);
     }
-    settle_block<W, false>(u, b, 0, e.set_rem);
+    {
+        SettlePrefix<W> f{u, b};
+        with_len<W, W - 1>(e.set_rem, f);
+    }
     if (__builtin_fabsf(u) < 2e-8f) return 0.0f / (float)e.accum;   /* kernel.cu:138 */
     ProductState st{1.0f, 0};
-    product_block<W, false>(u, st, b, e.head_lo, e.head_hi);
+    if (e.head_hi == e.B) {
+        /* the usual head: the rest of the block the settle ended in */
+        ProductHead<W> f{u, st, b, e.B};
+        with_len<W, W - 1>(e.head_lo, f);
+    } else {
+        product_block<W, false>(u, st, b, e.head_lo, e.head_hi);
+    }
     {
         ProductFull<W> f{u, st, b, e.acc_blocks};
         with_block_len<W, W>(e.B, f);
     }
-    product_block<W, false>(u, st, b, 0, e.acc_rem);
+    {
+        ProductPrefix<W> f{u, st, b};
+        with_len<W, W - 1>(e.acc_rem, f);
+    }
     float lg[4];
     lg[0] = __builtin_amdgcn_logf(__builtin_fabsf(x));
     lg[1] = __builtin_amdgcn_logf(__builtin_fabsf(y));
@@ -648,6 +734,41 @@ struct PlainFull {
     }
 };
 
+/* Plain steps at window positions [LO, HI), guarded past W/2 as in
+ * product_range; ACC accumulates the log of the derivative. */
+template <int W, bool ACC>
+struct PlainRange {
+    float &v;
+    float &s;
+    const float (&r)[W];
+    uint32_t hi;
+    template <int LO, int HI, bool GUARD>
+    __device__ __forceinline__ void run()
+    {
+#pragma unroll
+        for (int j = LO; j < HI; ++j) {
+            if (!GUARD || j <= W / 2 || (uint32_t)j < hi) {
+                v = r[j] * __builtin_fmaf(-v, v, v);
+                if (ACC) s += __builtin_amdgcn_logf(__builtin_fabsf(__builtin_fmaf(-2.0f * r[j], v, r[j])));
+            }
+        }
+    }
+};
+
+template <int W, bool ACC>
+struct PlainPrefix {
+    PlainRange<W, ACC> rg;
+    template <int N>
+    __device__ __forceinline__ void go() { rg.template run<0, N, false>(); }
+};
+
+template <int W, bool ACC>
+struct PlainHead {
+    PlainRange<W, ACC> rg;
+    template <int N>
+    __device__ __forceinline__ void go() { rg.template run<N, W, true>(); }
+};
+
 template <int W>
 __device__ __forceinline__ float lambda_plain_w(float x, float y, float z,
                                                 const EvalDesc &e)
@@ -663,14 +784,25 @@ __device__ __forceinline__ float lambda_plain_w(float x, float y, float z,
         PlainFull<W, false> f{v, s, r, e.set_blocks};
         with_block_len<W, W>(e.B, f);
     }
-    plain_block<W, false>(v, s, r, 0, e.set_rem, false);
+    {
+        PlainPrefix<W, false> f{{v, s, r, 0}};
+        with_len<W, W - 1>(e.set_rem, f);
+    }
     if (v == 0.5f) return 0.0f / (float)e.accum;
-    plain_block<W, false>(v, s, r, e.head_lo, e.head_hi, true);
+    if (e.head_hi == e.B) {
+        PlainHead<W, true> f{{v, s, r, e.B}};
+        with_len<W, W - 1>(e.head_lo, f);
+    } else {
+        plain_block<W, false>(v, s, r, e.head_lo, e.head_hi, true);
+    }
     {
         PlainFull<W, true> f{v, s, r, e.acc_blocks};
         with_block_len<W, W>(e.B, f);
     }
-    plain_block<W, false>(v, s, r, 0, e.acc_rem, true);
+    {
+        PlainPrefix<W, true> f{{v, s, r, 0}};
+        with_len<W, W - 1>(e.acc_rem, f);
+    }
     const float lam = s * kLn2f / (float)e.accum;
     return __builtin_isfinite(lam) ? lam : __builtin_nanf("");
 }
@@ -722,6 +854,48 @@ struct Fp64Full {
     }
 };
 
+/* FP64 steps at window positions [LO, HI), guarded past W/2 as in
+ * product_range; the product is renormalised after position 15 (W = 32)
+ * and at the end: exact power-of-two scalings, and the final (E, M) with M
+ * in [0.5, 1) is the same wherever the earlier ones fell. */
+template <int W, bool ACC>
+struct Fp64Range {
+    double &v;
+    ProductState64 &st;
+    const double (&r)[W];
+    const double (&m2r)[W];
+    uint32_t hi;
+    template <int LO, int HI, bool GUARD>
+    __device__ __forceinline__ void run()
+    {
+#pragma unroll
+        for (int j = LO; j < HI; ++j) {
+            if (!GUARD || j <= W / 2 || (uint32_t)j < hi) {
+                v = (r[j] * v) * (1.0 - v);
+                if (ACC) {
+                    st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
+                    if (W > 16 && j == 15) renorm64(st);
+                }
+            }
+        }
+        if (ACC) renorm64(st);
+    }
+};
+
+template <int W, bool ACC>
+struct Fp64Prefix {
+    Fp64Range<W, ACC> rg;
+    template <int N>
+    __device__ __forceinline__ void go() { rg.template run<0, N, false>(); }
+};
+
+template <int W, bool ACC>
+struct Fp64Head {
+    Fp64Range<W, ACC> rg;
+    template <int N>
+    __device__ __forceinline__ void go() { rg.template run<N, W, true>(); }
+};
+
 template <int W>
 __device__ __forceinline__ double lambda_fp64_w(float xf, float yf, float zf,
                                                 const EvalDesc &e)
@@ -742,15 +916,26 @@ __device__ __forceinline__ double lambda_fp64_w(float xf, float yf, float zf,
         Fp64Full<W, false> f{v, st, r, m2r, e.set_blocks};
         with_block_len<W, W>(e.B, f);
     }
-    fp64_block<W, false, false>(v, st, r, m2r, 0, e.set_rem);
+    {
+        Fp64Prefix<W, false> f{{v, st, r, m2r, 0}};
+        with_len<W, W - 1>(e.set_rem, f);
+    }
     const double off = v - 0.5;
     if (!(off <= -1e-8 || off >= 1e-8)) return 0.0 / (double)e.accum;
-    fp64_block<W, false, true>(v, st, r, m2r, e.head_lo, e.head_hi);
+    if (e.head_hi == e.B) {
+        Fp64Head<W, true> f{{v, st, r, m2r, e.B}};
+        with_len<W, W - 1>(e.head_lo, f);
+    } else {
+        fp64_block<W, false, true>(v, st, r, m2r, e.head_lo, e.head_hi);
+    }
     {
         Fp64Full<W, true> f{v, st, r, m2r, e.acc_blocks};
         with_block_len<W, W>(e.B, f);
     }
-    fp64_block<W, false, true>(v, st, r, m2r, 0, e.acc_rem);
+    {
+        Fp64Prefix<W, true> f{{v, st, r, m2r, 0}};
+        with_len<W, W - 1>(e.acc_rem, f);
+    }
     const double lam = ((double)st.E * kLn2 + log(st.M)) / (double)e.accum;
     return __builtin_isfinite(lam) ? lam : __builtin_nan("");
 }
