@@ -226,11 +226,12 @@ int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_points,
                      float *d_depth, float *d_lambda, uint32_t *d_evals,
                      const LyapRenderMode *mode);
 
-/* Diagnostic: a device buffer of 8 uint64 per wave slot, 8192 slots per
- * launch level, 4 levels (2 MiB), that later render launches fill with
+/* Diagnostic: a device buffer of 16 uint64 per wave slot, 8192 slots per
+ * launch level, 4 levels (4 MiB), that later render launches fill with
  * {rounds, rays started, start clock, end clock, clocks evaluating, clocks
- * refilling, clocks combining, 0}; NULL switches it off. No reference
- * counterpart. */
+ * refilling, clocks combining, clocks of those refilling spent claiming
+ * tiles, clocks of those combining spent handing rays over, 0 x 7}; NULL
+ * switches it off. No reference counterpart. */
 int lyap_debug_wave_log(lyap_ctx *ctx, void *d_log);
 
 /* Kernel timing: while on, every render kernel launch on the context's

@@ -75,10 +75,10 @@ struct RenderArgs {
     uint32_t tile_rays;
     uint32_t lanes_per_ray;    /* lanes marching one ray while rays remain */
     uint32_t slots;            /* samples each lane evaluates per round (1..4) */
-    uint32_t *tile_counter;
+    /* kTileHeads claim counters, kHeadStride words apart (own cache lines) */
+    uint32_t *tile_heads;
     unsigned long long *eval_counter;
-    /* optional per-wave record {rounds, rays, start clock, end clock,
-     * clocks in evaluation, clocks in refill/regroup, clocks in combine} */
+    /* optional per-wave record of kWaveLogWords words (see the kernel's end) */
     unsigned long long *wave_log;
     /* Launch level: 0 takes rays from pixel tiles; a later level takes the
      * rays the previous launch handed over. A ray that has taken `budget`
@@ -612,11 +612,23 @@ __device__ __forceinline__ int march_consume(const ViewDesc &v, Ray &r, float l,
  * first lane is gbase sits at index gbase * slots + j. */
 constexpr uint32_t kMaxSlots = 4;
 
+/* Tile claiming. Every wave's first tile is its wave index; the tiles after
+ * the first n_waves are dealt round-robin to kTileHeads counters (tile
+ * n_waves + kTileHeads * c + h is the c-th claim on head h), so claims spread
+ * over kTileHeads addresses instead of queueing on one. A block starts on head
+ * blockIdx % kTileHeads and moves on when its head runs dry. */
+constexpr uint32_t kTileHeads = 8;
+constexpr uint32_t kHeadStride = 32;          /* 128 bytes */
+constexpr uint32_t kWaveLogWords = 16;
+
 struct RoundStash {
     float lam[64 * kMaxSlots];
     float t[64 * kMaxSlots];
     float x[64 * kMaxSlots], y[64 * kMaxSlots], z[64 * kMaxSlots];
     uint32_t ok[64 * kMaxSlots];
+    /* a march sample's share of the chaos and attractor sums (0 when it adds
+     * to neither), formed by the lane that evaluated it */
+    float ca[64 * kMaxSlots], aa[64 * kMaxSlots];
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -628,9 +640,11 @@ __device__ __forceinline__ void wave_sync_lds()
 
 /* Combines the group's K samples (read from the round's stash at
  * base + j) and advances the replicated ray state in the sequential
- * method's order. `evm` has bit j set when sample j stops a march (opaque,
- * invalid, near/far toggle, jitter). All lanes run the same uniform loops.
- * Returns true when this lane's ray finished this round. */
+ * method's order. A group's lanes share one phase, and `evm` holds one bit
+ * per sample j for it: marching, set when sample j stops the march (opaque,
+ * invalid, near/far toggle, jitter); refining, set when sample j's lambda is
+ * above the opaque threshold. All lanes run the same uniform loops. Returns
+ * true when this lane's ray finished this round. */
 __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStash &st,
                         uint32_t base, uint32_t K, bool busy, bool leader)
 {
@@ -646,28 +660,31 @@ __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStas
     if (__ballot(ph == PH_MARCH) != 0) {
         const bool marching = ph == PH_MARCH;
         const uint32_t e = evm ? (uint32_t)__ffsll((unsigned long long)evm) - 1u : K;
-        /* transparent samples 0..e-1, summed in order (kernel.cu:370-375);
-         * four loads issued ahead of their sequential adds */
+        /* transparent samples 0..e-1 added in order (kernel.cu:370-375). Each
+         * sample's share was formed when it was evaluated; a sample that adds
+         * to neither sum holds +0, and adding +0 leaves a sum unchanged (the
+         * sums start at +0 and so are never -0), so the bits are those of the
+         * sequential method. Four loads are issued ahead of their adds. */
         for (uint32_t j0 = 0; __ballot(marching && j0 < e) != 0; j0 += 4u) {
-            float l4[4];
-#pragma unroll
-            for (uint32_t q = 0; q < 4u; ++q) l4[q] = st.lam[min(base + j0 + q, 64u * kMaxSlots - 1u)];
+            float c4[4], a4[4];
 #pragma unroll
             for (uint32_t q = 0; q < 4u; ++q) {
-                const float lj = l4[q];
-                if (marching && j0 + q < e) {
-                    if (r.index > 0) {
-                        const bool to_c = lj > v.prm.chaosThreshold;
-                        const bool to_a = !to_c && lj > v.prm.opaqueThreshold;
-                        r.c += to_c ? lj : 0.0f;
-                        r.a += to_a ? lj : 0.0f;
-                    }
-                    ++r.evals;
-                    ++r.launch_samples;
-                    ++r.index;
-                    r.l = lj;
-                }
+                const uint32_t i = min(base + j0 + q, 64u * kMaxSlots - 1u);
+                c4[q] = st.ca[i];
+                a4[q] = st.aa[i];
             }
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) {
+                const bool take = marching && j0 + q < e;
+                r.c = take ? r.c + c4[q] : r.c;
+                r.a = take ? r.a + a4[q] : r.a;
+            }
+        }
+        if (marching && e > 0) {
+            r.evals += e;
+            r.launch_samples += e;
+            r.index += e;
+            r.l = st.lam[min(base + e - 1u, 64u * kMaxSlots - 1u)];
         }
         /* position and lambda of sample e */
         const uint32_t src = min(base + (e < K ? e : K - 1u), 64u * kMaxSlots - 1u);
@@ -705,20 +722,25 @@ __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStas
         }
     }
 
-    /* bisection: follow the actual path through this round's tree levels */
+    /* bisection: follow the actual path through this round's tree levels,
+     * reading each node's side of the threshold from `evm`; the lambda of the
+     * last node on the opaque side is the one read from the stash */
     if (__ballot(ph == PH_REFINE && bisect) != 0) {
         const uint32_t m = (ph == PH_REFINE && bisect)
                                ? bisect_levels(K, (uint32_t)v.bisect_steps - r.k) : 0u;
-        uint32_t node = 0;
+        uint32_t node = 0, hi_node = 0xFFFFFFFFu;
         for (uint32_t i = 0; __ballot(i < m) != 0; ++i) {
-            const float ln = st.lam[min(base + node, 64u * kMaxSlots - 1u)];
             if (i < m) {
                 const float mid = r.lo + (r.hi - r.lo) * 0.5f;
                 ++r.evals;
                 ++r.k;
-                if (!(ln > thr)) { r.hi = mid; r.l_hi = ln; node = 2u * node + 1u; }
+                if (((evm >> node) & 1ull) == 0) { r.hi = mid; hi_node = node; node = 2u * node + 1u; }
                 else { r.lo = mid; node = 2u * node + 2u; }
             }
+        }
+        if (__ballot(hi_node != 0xFFFFFFFFu) != 0) {
+            const float ln = st.lam[min(base + (hi_node & 63u), 64u * kMaxSlots - 1u)];
+            if (hi_node != 0xFFFFFFFFu) r.l_hi = ln;
         }
         if (ph == PH_REFINE && bisect && (int)r.k >= v.bisect_steps) {
             r.t = r.hi;
@@ -821,6 +843,37 @@ __device__ __forceinline__ bool tile_lookup(const ViewDesc *__restrict__ views,
     return false;
 }
 
+/* Next tile from the claim counters (wave-uniform; every lane calls it), or
+ * 0xFFFFFFFF when every counter has run dry. `head` and `tried` persist
+ * across calls: the counter in use and how many were found dry. Once off its
+ * first counter a wave reads a counter before adding to it, so dry counters
+ * cost a load, not another queued atomic. */
+__device__ __forceinline__ uint32_t claim_tile(uint32_t *heads, uint32_t n_waves,
+                                               uint32_t level_tiles, uint32_t &head,
+                                               uint32_t &tried)
+{
+    while (tried < kTileHeads) {
+        const uint32_t first = n_waves + head;
+        const uint32_t cnt =
+            level_tiles > first ? (level_tiles - first + kTileHeads - 1u) / kTileHeads : 0u;
+        if (cnt > 0) {
+            uint32_t *h = heads + head * kHeadStride;
+            uint32_t c = cnt;
+            if (__lane_id() == 0) {
+                const uint32_t seen =
+                    tried == 0 ? 0u
+                               : __hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (seen < cnt) c = atomicAdd(h, 1u);
+            }
+            c = uniform((uint32_t)__shfl((int)c, 0));
+            if (c < cnt) return first + kTileHeads * c;
+        }
+        head = (head + 1u) % kTileHeads;
+        ++tried;
+    }
+    return 0xFFFFFFFFu;
+}
+
 template <int W, int PREC, bool MARGIN>
 __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restrict__ views,
                                                          RenderArgs a)
@@ -854,10 +907,25 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
     }
     if (K0 * S > 64u) K0 = 64u / S;
     K0 = uniform(K0);
+    /* tiles of this launch: pixel tiles, or each view's handed-over rays */
+    uint32_t level_tiles = a.total_tiles;
+    if (a.level > 0) {
+        level_tiles = 0;
+        for (uint32_t v = 0; v < a.n_views; ++v) {
+            const uint32_t cnt = uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+            level_tiles += (cnt + tile - 1u) / tile;
+        }
+    }
     uint32_t K = K0;                  /* lanes per ray; wave-uniform */
     const unsigned long long t_start = a.wave_log ? clock64() : 0ull;
-    unsigned long long c_eval = 0, c_refill = 0, c_combine = 0, c_mark = t_start;
+    unsigned long long c_eval = 0, c_refill = 0, c_combine = 0, c_claim = 0, c_hand = 0;
+    unsigned long long c_mark = t_start;
     uint32_t rounds = 0, rays_started = 0;
+    const uint32_t n_waves = gridDim.x * 4u;
+    const uint32_t wid = uniform(blockIdx.x * 4u + threadIdx.x / 64u);
+    uint32_t head = blockIdx.x % kTileHeads;  /* claim counter in use */
+    uint32_t heads_tried = 0;                 /* counters found dry */
+    bool first_tile = true;
     uint32_t cur_view = 0xFFFFFFFFu;  /* view the lanes work on */
     uint32_t pos = 0, end = 0;        /* unstarted rays of the current tile */
     uint32_t pend_tile = 0xFFFFFFFFu; /* claimed tile of another view */
@@ -891,11 +959,20 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
                     continue;
                 }
                 if (queue_empty) break;
-                uint32_t t = 0;
-                if (lane == 0) t = atomicAdd(a.tile_counter, 1u);
-                t = uniform((uint32_t)__shfl((int)t, 0));
+                const unsigned long long c0 = a.wave_log ? clock64() : 0ull;
+                uint32_t t;
+                if (first_tile) {
+                    first_tile = false;
+                    t = wid;
+                } else {
+                    t = claim_tile(a.tile_heads, n_waves, level_tiles, head, heads_tried);
+                }
+                if (a.wave_log) c_claim += clock64() - c0;
                 uint32_t tv, f, l;
-                if (!tile_lookup(views, a, t, tile, tv, f, l)) { queue_empty = true; break; }
+                if (t >= level_tiles || !tile_lookup(views, a, t, tile, tv, f, l)) {
+                    queue_empty = true;
+                    break;
+                }
                 if (tv != cur_view) { pend_tile = t; continue; }
                 pos = f;
                 end = l;
@@ -963,7 +1040,7 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
         const bool busy = r.phase != PH_IDLE;
         RoundStash &st = stash[threadIdx.x >> 6];
         const uint64_t kmask = K >= 64u ? ~0ull : ((1ull << K) - 1ull);
-        uint64_t evm = 0;                     /* the group's stopping samples */
+        uint64_t evm = 0;                     /* the group's sample bits (see combine) */
         for (uint32_t sl = 0; sl < S; ++sl) {
             const uint32_t q = sl * K + rank;
             float lam = NAN;
@@ -977,9 +1054,11 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
                     !(__builtin_fabsf(lam - v.prm.opaqueThreshold) >= v.margin))
                     lam = eval_fp64_call<W>(v.e, sp.x, sp.y, sp.z);
             }
-            /* does sample q stop the march (opaque, invalid, a near/far
-             * toggle, or jitter, whose step needs this lambda)? */
+            /* marching: does sample q stop the march (opaque, invalid, a
+             * near/far toggle, or jitter, whose step needs this lambda)?
+             * refining: is its lambda above the opaque threshold? */
             bool ev = true;
+            if (busy && r.phase == PH_REFINE) ev = lam > v.prm.opaqueThreshold;
             if (busy && r.phase == PH_MARCH && valid) {
                 const LyapParams &prm = v.prm;
                 const bool near_ = (r.flags & 1u) != 0;
@@ -998,6 +1077,15 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
                 st.y[idx] = sp.y;
                 st.z[idx] = sp.z;
                 st.ok[idx] = valid ? 1u : 0u;
+                /* march sample q has march index r.index + q; the first
+                 * sample of a ray adds to neither sum (kernel.cu:370-375) */
+                float sc = 0.0f, sa = 0.0f;
+                if (busy && r.phase == PH_MARCH && r.index + q > 0u) {
+                    if (lam > v.prm.chaosThreshold) sc = lam;
+                    else if (lam > v.prm.opaqueThreshold) sa = lam;
+                }
+                st.ca[idx] = sc;
+                st.aa[idx] = sa;
             }
         }
         wave_sync_lds();
@@ -1008,13 +1096,20 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
         }
         if (combine(v, r, evm, st, gbase * S, KS, busy, rank == 0) && rank == 0)
             done_evals += r.evals;
-        /* hand rays that used this launch's sample budget to the next launch */
+        /* hand rays that used this launch's sample budget to the next launch:
+         * one atomic per wave reserves the records of all its leaders */
         if (a.budget != 0) {
             const bool over = r.phase == PH_MARCH && r.launch_samples >= a.budget;
-            if (__ballot(over) != 0) {
+            const uint64_t om = __ballot(over && rank == 0);
+            if (om != 0) {
+                const unsigned long long h0 = a.wave_log ? clock64() : 0ull;
+                uint32_t base = 0;
+                if (lane == 0)
+                    base = atomicAdd(&a.out_counts[uniform(cur_view)], (uint32_t)__popcll(om));
+                base = uniform((uint32_t)__shfl((int)base, 0));
                 uint32_t ok = 0;
                 if (over && rank == 0) {
-                    const uint32_t slot = atomicAdd(&a.out_counts[uniform(cur_view)], 1u);
+                    const uint32_t slot = base + (uint32_t)__popcll(om & ((1ull << lane) - 1ull));
                     if (slot < v.cont_cap) {
                         ContEntry c;
                         suspend_ray(r, c);
@@ -1024,6 +1119,7 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
                 }
                 ok = (uint32_t)__shfl((int)ok, (int)gbase);
                 if (over && ok) r.phase = PH_IDLE;
+                if (a.wave_log) c_hand += clock64() - h0;
             }
         }
     }
@@ -1031,9 +1127,8 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
     unsigned long long sum = done_evals;
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
     if (lane == 0 && sum) atomicAdd(a.eval_counter, sum);
-    if (a.wave_log && lane == 0) {
-        const uint32_t wid = blockIdx.x * 4u + threadIdx.x / 64u;
-        unsigned long long *o = a.wave_log + 8ull * (wid + 8192ull * a.level);
+    if (a.wave_log && lane == 0 && wid < 8192u) {
+        unsigned long long *o = a.wave_log + kWaveLogWords * (wid + 8192ull * a.level);
         o[0] = rounds;
         o[1] = rays_started;
         o[2] = t_start;
@@ -1041,7 +1136,9 @@ __global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restr
         o[4] = c_eval;
         o[5] = c_refill;
         o[6] = c_combine;
-        o[7] = 0;
+        o[7] = c_claim;
+        o[8] = c_hand;
+        for (uint32_t k = 9; k < kWaveLogWords; ++k) o[k] = 0;
     }
 }
 
@@ -1185,12 +1282,16 @@ static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, const LyapTuni
 /* Device and pinned memory one launch sequence uses. The context owns one
  * set for its direct calls; a captured frame owns its own, so a replay never
  * sees memory that a later call reallocated. */
+/* Words of the counter block: the evaluation count in the first cache line,
+ * then kTileHeads claim counters per launch level. */
+constexpr uint32_t kCounterWords = kHeadStride + 4u * kTileHeads * kHeadStride;
+
 struct RenderResources {
     void *view_table = nullptr;      /* device ViewDesc[] */
     size_t view_table_bytes = 0;
     void *staging = nullptr;         /* pinned host copy of the table */
     size_t staging_bytes = 0;
-    uint32_t *counters = nullptr;    /* [2..3] evaluations, [4..7] tile heads */
+    uint32_t *counters = nullptr;    /* kCounterWords: [2..3] evaluations, then tile heads */
     void *cont_buf = nullptr;        /* rays handed between levels */
     size_t cont_buf_bytes = 0;
     void *cont_counts = nullptr;     /* per view and level */
@@ -1253,7 +1354,7 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
     if (rc) return rc;
     if (!res.counters) {
         void *c = nullptr;
-        rc = ctx_alloc(ctx, &c, 256);
+        rc = ctx_alloc(ctx, &c, sizeof(uint32_t) * kCounterWords);
         if (rc) return rc;
         res.counters = (uint32_t *)c;
     }
@@ -1279,7 +1380,7 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
 {
     LYAP_HIP(ctx, hipMemcpyAsync(res.view_table, res.staging, sizeof(ViewDesc) * n_views,
                                  hipMemcpyHostToDevice, ctx->stream));
-    int rc = launch_zero_u32(ctx, res.counters, 8);
+    int rc = launch_zero_u32(ctx, res.counters, kCounterWords);
     if (rc) return rc;
     rc = launch_zero_u32(ctx, (uint32_t *)res.cont_counts, 3 * n_views);
     if (rc) return rc;
@@ -1304,7 +1405,7 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         a.tile_rays = plan.tile_rays;
         a.lanes_per_ray = plan.lanes_per_ray;
         a.slots = plan.slots;
-        a.tile_counter = res.counters + 4 + level;
+        a.tile_heads = res.counters + kHeadStride + level * kTileHeads * kHeadStride;
         a.eval_counter = (unsigned long long *)(res.counters + 2);
         a.wave_log = (unsigned long long *)ctx->debug_wave_log;
         a.level = level;
