@@ -874,9 +874,13 @@ __device__ __forceinline__ uint32_t claim_tile(uint32_t *heads, uint32_t n_waves
     return 0xFFFFFFFFu;
 }
 
+/* Three waves per SIMD (at most 168 VGPRs): the 16-slot FP32 kernels fit
+ * without spilling; the 32-slot, FP64 and margin kernels would otherwise
+ * take 200+ VGPRs and run two waves, and measure 20-30% slower that way than
+ * with the few spills this target costs them. */
 template <int W, int PREC, bool MARGIN>
-__global__ __launch_bounds__(256) void render_persistent(const ViewDesc *__restrict__ views,
-                                                         RenderArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
 {
     __shared__ RoundStash stash[4];   /* each wave's samples of the round */
     const uint32_t lane = __lane_id();
