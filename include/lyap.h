@@ -230,8 +230,9 @@ int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_points,
  * launch level, 4 levels (4 MiB), that later render launches fill with
  * {rounds, rays started, start clock, end clock, clocks evaluating, clocks
  * refilling, clocks combining, clocks of those refilling spent claiming
- * tiles, clocks of those combining spent handing rays over, 0 x 7}; NULL
- * switches it off. No reference counterpart. */
+ * tiles, clocks of those combining spent handing rays over, clocks of
+ * those refilling spent starting rays, 0 x 6}; NULL switches it off. No
+ * reference counterpart. */
 int lyap_debug_wave_log(lyap_ctx *ctx, void *d_log);
 
 /* Kernel timing: while on, every render kernel launch on the context's

@@ -923,6 +923,7 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
     uint32_t K = K0;                  /* lanes per ray; wave-uniform */
     const unsigned long long t_start = a.wave_log ? clock64() : 0ull;
     unsigned long long c_eval = 0, c_refill = 0, c_combine = 0, c_claim = 0, c_hand = 0;
+    unsigned long long c_start = 0;
     unsigned long long c_mark = t_start;
     uint32_t rounds = 0, rays_started = 0;
     const uint32_t n_waves = gridDim.x * 4u;
@@ -988,6 +989,7 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             const uint32_t gb = lane - rank;
             const bool gneed = ((mask >> gb) & 1ull) != 0;
             const uint32_t rk = (uint32_t)__popcll(mask & ((1ull << gb) - 1ull));
+            const unsigned long long s0 = a.wave_log ? clock64() : 0ull;
             if (gneed && rk < avail) {
                 const ViewDesc &v = views[uniform(cur_view)];
                 if (a.level == 0) {
@@ -998,6 +1000,7 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
                     started = true;
                 }
             }
+            if (a.wave_log) c_start += clock64() - s0;
             const uint32_t taken = (uint32_t)__popcll(mask);
             pos += taken < avail ? taken : avail;
         }
@@ -1142,7 +1145,8 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
         o[6] = c_combine;
         o[7] = c_claim;
         o[8] = c_hand;
-        for (uint32_t k = 9; k < kWaveLogWords; ++k) o[k] = 0;
+        o[9] = c_start;
+        for (uint32_t k = 10; k < kWaveLogWords; ++k) o[k] = 0;
     }
 }
 
@@ -1206,7 +1210,7 @@ struct LaunchPlan {
 static const uint32_t kDefaultSlots = 1u;
 
 /* March-sample budgets of the launch levels; the last level has none. */
-static const uint32_t kLevelBudget[4] = {32u, 64u, 192u, 0u};
+static const uint32_t kLevelBudget[4] = {32u, 64u, 128u, 0u};
 
 /* Level 0 hands rays over sooner when several lanes march each ray (a small
  * launch, where the long rays decide the frame time); with one lane per ray
