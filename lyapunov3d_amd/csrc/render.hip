@@ -241,11 +241,34 @@ struct Ray {
     F3 V, P;
     float t, t1, dt, Fdt, Ndt, last_step;
     float a, c, l;
-    float lo, hi, l_hi;    /* bisection */
-    float Qdt, Qt0, Qt1, minQdt;
-    F3 QdV;
-    float h;
-    float gx, gy, gz;      /* central differences */
+    /* phase storage, one set live at a time: the reversing walk's
+     * {Qdt, Qt0, Qt1, minQdt, QdV} in q[0..6], the bisection's {lo, hi, l_hi}
+     * in q[0..2], the normal's {h, gx, gy, gz} (central differences) in q[3..6] */
+    float q[7];
+    __device__ float &lo() { return q[0]; }
+    __device__ float lo() const { return q[0]; }
+    __device__ float &hi() { return q[1]; }
+    __device__ float hi() const { return q[1]; }
+    __device__ float &l_hi() { return q[2]; }
+    __device__ float l_hi() const { return q[2]; }
+    __device__ float &Qdt() { return q[0]; }
+    __device__ float Qdt() const { return q[0]; }
+    __device__ float &Qt0() { return q[1]; }
+    __device__ float Qt0() const { return q[1]; }
+    __device__ float &Qt1() { return q[2]; }
+    __device__ float Qt1() const { return q[2]; }
+    __device__ float &minQdt() { return q[3]; }
+    __device__ float minQdt() const { return q[3]; }
+    __device__ F3 QdV() const { return {q[4], q[5], q[6]}; }
+    __device__ void set_QdV(F3 d) { q[4] = d.x; q[5] = d.y; q[6] = d.z; }
+    __device__ float &h() { return q[3]; }
+    __device__ float h() const { return q[3]; }
+    __device__ float &gx() { return q[4]; }
+    __device__ float gx() const { return q[4]; }
+    __device__ float &gy() { return q[5]; }
+    __device__ float gy() const { return q[5]; }
+    __device__ float &gz() { return q[6]; }
+    __device__ float gz() const { return q[6]; }
     uint32_t launch_samples;   /* march samples taken in this launch */
     /* direct sampling (bisection method): t = tb + m * dtb, rebased when the
      * step length changes */
@@ -271,12 +294,8 @@ __device__ __forceinline__ Ray shuffle_ray(const Ray &r, int src)
     o.t = shf(r.t, src); o.t1 = shf(r.t1, src); o.dt = shf(r.dt, src);
     o.Fdt = shf(r.Fdt, src); o.Ndt = shf(r.Ndt, src); o.last_step = shf(r.last_step, src);
     o.a = shf(r.a, src); o.c = shf(r.c, src); o.l = shf(r.l, src);
-    o.lo = shf(r.lo, src); o.hi = shf(r.hi, src); o.l_hi = shf(r.l_hi, src);
-    o.Qdt = shf(r.Qdt, src); o.Qt0 = shf(r.Qt0, src); o.Qt1 = shf(r.Qt1, src);
-    o.minQdt = shf(r.minQdt, src);
-    o.QdV = {shf(r.QdV.x, src), shf(r.QdV.y, src), shf(r.QdV.z, src)};
-    o.h = shf(r.h, src);
-    o.gx = shf(r.gx, src); o.gy = shf(r.gy, src); o.gz = shf(r.gz, src);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) o.q[i] = shf(r.q[i], src);
     o.launch_samples = shu(r.launch_samples, src);
     o.tb = shf(r.tb, src); o.dtb = shf(r.dtb, src); o.m = shu(r.m, src);
     return o;
@@ -352,25 +371,34 @@ __device__ void write_hit(const ViewDesc &v, const Ray &r, F3 N)
 /* Starts ray `id` of view v; returns false (after the leader wrote a miss,
  * or nothing for a padding ray past the image edge) when no evaluation is
  * needed. Every lane of the ray's group runs it and gets the same state. */
-__device__ bool start_ray(const ViewDesc &v, uint32_t id, Ray &r, bool leader)
+enum : int { RAY_PAD = 0, RAY_MISS = 1, RAY_HIT = 2 };
+
+/* Pixel, direction and box interval of ray `id` of view v (pixel mapping
+ * kernel.cu:500-516, direction :160-163, box clip :180-279). RAY_PAD for a
+ * padding ray past the image edge. */
+__device__ __forceinline__ int ray_setup(const ViewDesc &v, uint32_t id, uint32_t &pix, F3 &V,
+                                         float &t0, float &t1)
 {
     id += v.ray_begin;
     const uint32_t blk = id >> 6, in = id & 63u;
     const uint32_t px = (blk % v.blocks_x) * 8u + (in & 7u);
     const uint32_t py = (blk / v.blocks_x) * 8u + (in >> 3);
-    if (px >= v.w || py >= v.h) return false;
-    r.pix = px + py * v.w;
-    r.evals = 0;
-    const F3 C = f3(v.cam.C);
-    F3 V = add(add(f3(v.cam.S0), mul(f3(v.cam.SDX), (float)px)), mul(f3(v.cam.SDY), (float)py));
+    if (px >= v.w || py >= v.h) return RAY_PAD;
+    pix = px + py * v.w;
+    V = add(add(f3(v.cam.S0), mul(f3(v.cam.SDX), (float)px)), mul(f3(v.cam.SDY), (float)py));
     V = normalized(V);
     V = divv(V, v.cam.M);
+    return clip_box(f3(v.cam.C), V, v.prm.lMin, v.prm.lMax, t0, t1) ? RAY_HIT : RAY_MISS;
+}
+/* The march state of a ray that meets the box over [t0, t1]
+ * (kernel.cu:281-314). Every lane of the ray's group runs it. */
+__device__ __forceinline__ void start_hit(const ViewDesc &v, uint32_t pix, F3 V, float t0,
+                                          float t1, Ray &r)
+{
+    r.pix = pix;
+    r.evals = 0;
     r.V = V;
-    float t0, t1;
-    if (!clip_box(C, V, v.prm.lMin, v.prm.lMax, t0, t1)) {
-        if (leader) write_miss(v, r);
-        return false;
-    }
+    const F3 C = f3(v.cam.C);
     r.t = t0;
     r.t1 = t1;
     r.P = add(C, mul(V, t0));
@@ -390,7 +418,6 @@ __device__ bool start_ray(const ViewDesc &v, uint32_t id, Ray &r, bool leader)
     r.dtb = r.dt;
     r.m = 0;
     r.phase = PH_MARCH;
-    return true;
 }
 
 /* Resumes a ray handed over by the previous launch. */
@@ -454,14 +481,14 @@ __device__ __forceinline__ void enter_normal(const ViewDesc &v, Ray &r)
 {
     r.phase = PH_NORMAL;
     r.k = 0;
-    r.h = (v.normal_h > 0.0f) ? v.normal_h : r.dt * v.prm.gradient;
+    r.h() = (v.normal_h > 0.0f) ? v.normal_h : r.dt * v.prm.gradient;
 }
 
 /* walk loop test (kernel.cu:420) */
 __device__ __forceinline__ bool walk_continues(const Ray &r)
 {
-    return r.k < kMaxWalk && r.t <= r.Qt1 && r.t >= r.Qt0 &&
-           (r.Qdt <= -r.minQdt || r.Qdt >= r.minQdt);
+    return r.k < kMaxWalk && r.t <= r.Qt1() && r.t >= r.Qt0() &&
+           (r.Qdt() <= -r.minQdt() || r.Qdt() >= r.minQdt());
 }
 
 __device__ __forceinline__ void enter_refine(const ViewDesc &v, Ray &r)
@@ -469,17 +496,17 @@ __device__ __forceinline__ void enter_refine(const ViewDesc &v, Ray &r)
     r.phase = PH_REFINE;
     r.k = 0;
     if (v.refine == LYAP_REFINE_BISECT) {
-        r.lo = r.t - r.last_step;
-        r.hi = r.t;
-        r.l_hi = r.l;
+        r.lo() = r.t - r.last_step;
+        r.hi() = r.t;
+        r.l_hi() = r.l;
         if (v.bisect_steps <= 0) enter_normal(v, r);
     } else {
         r.flags &= ~2u;
-        r.Qdt = r.dt * -0.5f;
-        r.QdV = mul(r.V, r.Qdt);
-        r.Qt1 = r.t;
-        r.Qt0 = r.t - r.dt;
-        r.minQdt = r.dt / v.prm.refine;
+        r.Qdt() = r.dt * -0.5f;
+        r.set_QdV(mul(r.V, r.Qdt()));
+        r.Qt1() = r.t;
+        r.Qt0() = r.t - r.dt;
+        r.minQdt() = r.dt / v.prm.refine;
         if (!walk_continues(r)) enter_normal(v, r);
     }
 }
@@ -534,7 +561,7 @@ __device__ __forceinline__ bool sample_point(const ViewDesc &v, const Ray &r,
             /* node rank of the next m levels: depth d, path bits below the top */
             const uint32_t n1 = rank + 1u;
             const uint32_t d = 31u - (uint32_t)__builtin_clz(n1);
-            float lo = r.lo, hi = r.hi;
+            float lo = r.lo(), hi = r.hi();
             for (int i = (int)d - 1; i >= 0; --i) {
                 const float mid = lo + (hi - lo) * 0.5f;
                 if ((n1 >> i) & 1u) lo = mid;   /* transparent branch */
@@ -545,14 +572,14 @@ __device__ __forceinline__ bool sample_point(const ViewDesc &v, const Ray &r,
             return true;
         }
         if (rank != 0) return false;
-        s = add(r.P, r.QdV);                            /* walk: t += Qdt; P += QdV */
+        s = add(r.P, r.QdV());                            /* walk: t += Qdt; P += QdV */
         return true;
     }
     /* PH_NORMAL: samples k.. in the order x-, x+, y-, y+, z-, z+ */
     const uint32_t n = r.k + rank;
     if (n >= 6) return false;
     s = r.P;
-    const float off = (n & 1u) ? r.h : -r.h;
+    const float off = (n & 1u) ? r.h() : -r.h();
     if (n < 2) s.x = r.P.x + off;
     else if (n < 4) s.y = r.P.y + off;
     else s.z = r.P.z + off;
@@ -731,21 +758,21 @@ __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStas
         uint32_t node = 0, hi_node = 0xFFFFFFFFu;
         for (uint32_t i = 0; __ballot(i < m) != 0; ++i) {
             if (i < m) {
-                const float mid = r.lo + (r.hi - r.lo) * 0.5f;
+                const float mid = r.lo() + (r.hi() - r.lo()) * 0.5f;
                 ++r.evals;
                 ++r.k;
-                if (((evm >> node) & 1ull) == 0) { r.hi = mid; hi_node = node; node = 2u * node + 1u; }
-                else { r.lo = mid; node = 2u * node + 2u; }
+                if (((evm >> node) & 1ull) == 0) { r.hi() = mid; hi_node = node; node = 2u * node + 1u; }
+                else { r.lo() = mid; node = 2u * node + 2u; }
             }
         }
         if (__ballot(hi_node != 0xFFFFFFFFu) != 0) {
             const float ln = st.lam[min(base + (hi_node & 63u), 64u * kMaxSlots - 1u)];
-            if (hi_node != 0xFFFFFFFFu) r.l_hi = ln;
+            if (hi_node != 0xFFFFFFFFu) r.l_hi() = ln;
         }
         if (ph == PH_REFINE && bisect && (int)r.k >= v.bisect_steps) {
-            r.t = r.hi;
+            r.t = r.hi();
             r.P = add(f3(v.cam.C), mul(r.V, r.t));
-            r.l = r.l_hi;
+            r.l = r.l_hi();
             enter_normal(v, r);
         }
     }
@@ -754,8 +781,8 @@ __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStas
     if (__ballot(ph == PH_REFINE && !bisect) != 0) {
         const float l0 = st.lam[min(base, 64u * kMaxSlots - 1u)];
         if (ph == PH_REFINE && !bisect) {
-            r.t += r.Qdt;
-            r.P = add(r.P, r.QdV);
+            r.t += r.Qdt();
+            r.P = add(r.P, r.QdV());
             ++r.evals;
             ++r.k;
             r.l = l0;
@@ -767,8 +794,8 @@ __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStas
                 if (sign) r.flags |= 2u;
                 else r.flags &= ~2u;
                 if (sign != osign) {
-                    r.Qdt *= -0.5f;
-                    r.QdV = mul(r.QdV, -0.5f);
+                    r.Qdt() *= -0.5f;
+                    r.set_QdV(mul(r.QdV(), -0.5f));
                 }
                 if (!walk_continues(r)) enter_normal(v, r);
             }
@@ -787,15 +814,15 @@ __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStas
             if (j < n6 && ph == PH_NORMAL && r.k < 6u) {
                 const uint32_t n = r.k;
                 const float sl = (n & 1u) ? lj : -lj;
-                if (n < 2) r.gx = (n == 0) ? sl : r.gx + sl;
-                else if (n < 4) r.gy = (n == 2) ? sl : r.gy + sl;
-                else r.gz = (n == 4) ? sl : r.gz + sl;
+                if (n < 2) r.gx() = (n == 0) ? sl : r.gx() + sl;
+                else if (n < 4) r.gy() = (n == 2) ? sl : r.gy() + sl;
+                else r.gz() = (n == 4) ? sl : r.gz() + sl;
                 ++r.k;
                 ++r.evals;
             }
         }
         if (ph == PH_NORMAL && r.k >= 6u) {
-            const F3 N = normalized({r.gx, r.gy, r.gz});
+            const F3 N = normalized({r.gx(), r.gy(), r.gz()});
             if (leader) write_hit(v, r, N);
             r.phase = PH_IDLE;
             finished = true;
@@ -943,17 +970,23 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             c_mark = now;
         }
         /* ---- refill idle groups (one ray per group of K lanes) ---------- */
+        /* groups are handed rays first and start them after the loop, so
+         * the ray state is written once per refill */
         bool started = false;
+        bool pend = false;            /* the group was given a ray this refill */
+        uint32_t p_id = 0, p_pix = 0; /* its handed-over record, or its pixel */
+        F3 p_V = {0.f, 0.f, 0.f};
+        float p_t0 = 0.f, p_t1 = 0.f;
         for (;;) {
             /* the reference's shape: a wave's rays all end before new ones start */
-            if (a.direct && __ballot(r.phase != PH_IDLE) != 0) break;
+            if (a.direct && __ballot(r.phase != PH_IDLE || pend) != 0) break;
             const uint32_t rank = lane % K;
-            const bool need = rank == 0 && lane + K <= 64u && r.phase == PH_IDLE;
+            const bool need = rank == 0 && lane + K <= 64u && r.phase == PH_IDLE && !pend;
             const uint64_t mask = __ballot(need);
             if (mask == 0) break;
             if (pos >= end) {
                 if (pend_tile != 0xFFFFFFFFu) {
-                    if (__ballot(r.phase != PH_IDLE) != 0) break;   /* drain first */
+                    if (__ballot(r.phase != PH_IDLE || pend) != 0) break;   /* drain first */
                     uint32_t tv, f, l;
                     tile_lookup(views, a, pend_tile, tile, tv, f, l);
                     pend_tile = 0xFFFFFFFFu;
@@ -993,16 +1026,27 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             if (gneed && rk < avail) {
                 const ViewDesc &v = views[uniform(cur_view)];
                 if (a.level == 0) {
-                    if (start_ray(v, pos + rk, r, rank == 0)) started = true;
-                    else r.phase = PH_IDLE;
+                    const int k = ray_setup(v, pos + rk, p_pix, p_V, p_t0, p_t1);
+                    if (k == RAY_HIT) pend = true;
+                    else if (k == RAY_MISS && rank == 0) {
+                        Ray m;
+                        m.pix = p_pix;
+                        m.evals = 0;
+                        write_miss(v, m);
+                    }
                 } else {
-                    resume_ray(a.cont_in[v.cont_base + pos + rk], r);
-                    started = true;
+                    p_id = v.cont_base + pos + rk;
+                    pend = true;
                 }
             }
             if (a.wave_log) c_start += clock64() - s0;
             const uint32_t taken = (uint32_t)__popcll(mask);
             pos += taken < avail ? taken : avail;
+        }
+        if (pend) {
+            if (a.level == 0) start_hit(views[uniform(cur_view)], p_pix, p_V, p_t0, p_t1, r);
+            else resume_ray(a.cont_in[p_id], r);
+            started = true;
         }
         rays_started += (uint32_t)__popcll(__ballot(started && lane % K == 0u));
         const uint64_t busy_mask = __ballot(r.phase != PH_IDLE);
