@@ -1051,6 +1051,13 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
         rays_started += (uint32_t)__popcll(__ballot(started && lane % K == 0u));
         const uint64_t busy_mask = __ballot(r.phase != PH_IDLE);
         if (busy_mask == 0) {
+            /* a refill that left every group idle (the wave's last one, or
+             * tiles whose rays all missed the box) is refill time too */
+            if (a.wave_log) {
+                const unsigned long long now = clock64();
+                c_refill += now - c_mark;
+                c_mark = now;
+            }
             if (queue_empty && pend_tile == 0xFFFFFFFFu) break;
             continue;
         }
