@@ -116,24 +116,64 @@ class Timer:
         return s.elapsed_time(e) / reps
 
 
+def _host_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cam, prm, seq, lights, n_lights, w, h, mode, budget_s):
-    """The oracle's FP32 as-written render on every k-th row, timed here."""
+    """The oracle's FP32 as-written render on every k-th row, timed here; the
+    `detail` object adds the FP64 form of the same frame on fewer rows, the
+    64^3 volume in both forms and z-slices of the 256^3 volume (each timed
+    once, a few seconds in all)."""
     from oracle import oracle as O
+    from tests import workloads as W
     threads = O.threads()
+
+    def rows_for(lam, budget):
+        t = time.perf_counter()
+        O.render(cam, prm, seq, lights, n_lights, w, h, mode=mode, lam=lam, rows=(h // 2, h), nthreads=threads)
+        one_row = max(time.perf_counter() - t, 1e-4)
+        n_rows = int(max(1, min(h, budget / one_row)))
+        step = max(1, h // n_rows)
+        rows = (step // 2, step)
+        t = time.perf_counter()
+        O.render(cam, prm, seq, lights, n_lights, w, h, mode=mode, lam=lam, rows=rows, nthreads=threads)
+        dt = time.perf_counter() - t
+        return rows, len(range(rows[0], h, rows[1])) * w, dt
+
+    rows, sampled, dt = rows_for(O.F32W, budget_s)
+    detail = {"host": _host_model(), "os_cpu_count": os.cpu_count(), "omp_threads": threads}
+    r64, s64, d64 = rows_for(O.F64, budget_s / 4)
+    detail["c2_fp64_rays_per_s"] = s64 / d64
+    detail["c2_fp64_sample"] = f"rows {r64[0]}::{r64[1]} ({s64} rays), {d64:.1f} s"
+    vprm, vseq, lo, hi, n = W.volume_config("c1")
+    for name, lam in (("fp32_as_written", O.F32W), ("fp64", O.F64)):
+        t = time.perf_counter()
+        O.volume(vprm, vseq, lo, hi, n, mode=lam, nthreads=threads)
+        detail[f"c1_volume_{name}_evals_per_s"] = n ** 3 / (time.perf_counter() - t)
+    vprm, vseq, lo, hi, n = W.volume_config("c3")
     t = time.perf_counter()
-    O.render(cam, prm, seq, lights, n_lights, w, h, mode=mode, lam=O.F32W, rows=(h // 2, h), nthreads=threads)
-    one_row = max(time.perf_counter() - t, 1e-4)
-    n_rows = int(max(1, min(h, budget_s / one_row)))
-    step = max(1, h // n_rows)
-    rows = (step // 2, step)
+    O.volume(vprm, vseq, lo, hi, n, mode=O.F32W, z_range=(n // 2, n // 2 + 1), nthreads=threads)
+    one = max(time.perf_counter() - t, 1e-4)
+    nz = int(max(1, min(n, (budget_s / 4) / one)))
+    z0 = (n - nz) // 2
     t = time.perf_counter()
-    O.render(cam, prm, seq, lights, n_lights, w, h, mode=mode, lam=O.F32W, rows=rows, nthreads=threads)
-    dt = time.perf_counter() - t
-    sampled = len(range(rows[0], h, rows[1])) * w
+    O.volume(vprm, vseq, lo, hi, n, mode=O.F32W, z_range=(z0, z0 + nz), nthreads=threads)
+    dv = time.perf_counter() - t
+    detail["c3_volume_fp32_as_written_evals_per_s"] = nz * n * n / dv
+    detail["c3_volume_sample"] = f"z-slices {z0}..{z0 + nz - 1} of 256 ({nz * n * n} voxels), {dv:.1f} s"
     return {
         "value": sampled / dt, "unit": "rays/s", "cores": threads, "kind": "port",
         "sample": f"oracle FP32-as-written render of rows {rows[0]}::{rows[1]} "
                   f"({sampled} rays) of the same frame, {dt:.1f} s on {threads} OpenMP threads",
+        "detail": detail,
     }
 
 
