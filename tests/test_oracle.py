@@ -230,6 +230,51 @@ def test_miss_record():
     assert np.all(np.isnan(out["points"][..., 6])) and np.all(np.isnan(out["points"][..., 8]))
 
 
+@pytest.mark.parametrize("lam_mode", [O.F64, O.F32W])
+def test_march_sums_in_order(lam_mode):
+    """The chaos and attractor sums of a hit (kernel.cu:370-375): each
+    transparent march sample after the first adds its lambda, in float and in
+    march order, to c when above chaosThreshold, else to a; the first sample
+    and the opaque one add to neither. On the analytic plane's centre ray the
+    bisection method's samples sit at t = t0 + m*dt, P = C + V*t in float,
+    so the sums are rebuilt here sample by sample from lyap4d at those points
+    (pinned by its own closed forms). The chaos threshold splits the samples
+    between the sums, so a reordered sum, a first sample counted, the opaque
+    sample counted or the classes swapped all change the bits."""
+    w = h = 24
+    cam, prm, lights, n, mode = analytic_scene(w, h, lyap.REFINE_BISECT)
+    prm.chaosThreshold = -0.25
+    sx = sy = 12
+    ok, V, t0, t1 = O.ray_setup(cam, prm, sx, sy)
+    assert ok
+    f = np.float32
+    Cv = np.array([cam.C.x, cam.C.y, cam.C.z], dtype=np.float32)
+    dt = f(f(f(t1) - f(t0)) / f(prm.depth))
+    a = c = f(0.0)
+    n_a = n_c = 0
+    m = 0
+    while True:
+        m += 1
+        t = f(f(t0) + f(f(m) * dt))
+        assert t <= f(t1)
+        P = (Cv + (V * t).astype(np.float32)).astype(np.float32)
+        lam = f(O.lyap_points(P[None, :], prm.d, prm.settle, prm.accum, "A", mode=lam_mode)[0])
+        if not lam > f(prm.opaqueThreshold):
+            break
+        if lam > f(prm.chaosThreshold):
+            c = f(c + lam)
+            n_c += 1
+        else:
+            a = f(a + lam)
+            n_a += 1
+    assert n_a >= 10 and n_c >= 10                    # both sums carry many terms
+    out = O.render(cam, prm, "A", lights, n, w, h, mode=mode, lam=lam_mode, rows=(sy, h))
+    assert out["hit_index"][sy, sx] == m
+    pt = out["points"][sy, sx]
+    assert pt[6].view(np.uint32) == a.view(np.uint32)
+    assert pt[7].view(np.uint32) == c.view(np.uint32)
+
+
 # --------------------------------------------------------------------- shade
 
 def test_shade_closed_form():
