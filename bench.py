@@ -555,6 +555,11 @@ def main():
                 fms = timer.run(lambda: bufs.render(ctx, cam, prm, seq, ld, n_lights, m), 10)
                 c2[name] = {"ms": fms, "rays_per_s": rays / (fms * 1e-3),
                             "lambda_evals_per_s": ctx.last_evals() / (fms * 1e-3)}
+            # a light edit: the frame's points re-shaded instead of re-rendered
+            bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
+            rms = timer.run(lambda: ctx.shade_points(bufs.rgba.data_ptr(), bufs.points.data_ptr(), cam,
+                                                     ld.data_ptr(), n_lights, w, h), 50)
+            c2["reshade_for_new_lights"] = {"ms": rms, "pixels_per_s": rays / (rms * 1e-3)}
             extra["c2_variants"] = c2
             log("c2 variants done")
         extra.update(bench_volumes(lyap, timer, ctx, which))

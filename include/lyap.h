@@ -226,6 +226,17 @@ int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_points,
                      float *d_depth, float *d_lambda, uint32_t *d_evals,
                      const LyapRenderMode *mode);
 
+/* Re-shades a rendered frame from its LyapPoint buffer (d_points as
+ * lyap_calc_render wrote it) for new lights, without marching: with the
+ * frame's own camera the RGBA equals what lyap_calc_render gives with those
+ * lights (kernel.cu:511-514). For the viewer's light edits, which raise
+ * update_lights/update_render but not update_calc and leave the camera's C
+ * and Q alone (lyap_interactive.cu:267-307, 340-356); the reference
+ * re-renders the whole frame for them (:124-142, 711). */
+int lyap_shade_points(lyap_ctx *ctx, LyapRGBA *d_rgba, const LyapPoint *d_points,
+                      const LyapCam *cam, const LyapLight *d_lights,
+                      unsigned num_lights, unsigned w, unsigned h);
+
 /* Diagnostic: a device buffer of 16 uint64 per wave slot, 8192 slots per
  * launch level, 4 levels (4 MiB), that later render launches fill with
  * {rounds, rays started, start clock, end clock, clocks evaluating, clocks

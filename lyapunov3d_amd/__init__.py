@@ -242,6 +242,7 @@ _SIGS = {
     "lyap_calc_points_halves": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.POINTER(LyapParams),
                                           C.POINTER(C.c_int32), C.c_uint]),
     "lyap_calc_render": (C.c_int, [_P, _P, _P, C.POINTER(LyapCam), C.POINTER(LyapParams), C.POINTER(C.c_int32), _P, C.c_uint, C.c_uint, C.c_uint, _P, _P, _P, C.POINTER(RenderMode)]),
+    "lyap_shade_points": (C.c_int, [_P, _P, _P, C.POINTER(LyapCam), _P, C.c_uint, C.c_uint, C.c_uint]),
     "lyap_last_evals": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "lyap_debug_wave_log": (C.c_int, [_P, _P]),
     "lyap_timing_enable": (C.c_int, [_P, C.c_int]),
@@ -494,6 +495,13 @@ class Context:
                                        depth_ptr or None, lambda_ptr or None, evals_ptr or None,
                                        C.byref(mode) if mode is not None else None)
         _check(rc, self.h, "lyap_calc_render")
+
+    def shade_points(self, rgba_ptr: int, points_ptr: int, cam: LyapCam, lights_ptr: int,
+                     num_lights: int, w: int, h: int):
+        """RGBA of a rendered frame's LyapPoint buffer under (new) lights."""
+        rc = self.lib.lyap_shade_points(self.h, rgba_ptr or None, points_ptr or None, C.byref(cam),
+                                        lights_ptr or None, num_lights, w, h)
+        _check(rc, self.h, "lyap_shade_points")
 
     def timing(self, on: bool):
         """Brackets every render kernel launch with HIP events while on."""

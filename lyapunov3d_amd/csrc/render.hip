@@ -1716,6 +1716,51 @@ extern "C" int lyap_calc_render(lyap_ctx *ctx, LyapRGBA *d_rgba, LyapPoint *d_po
                          w, h, d_depth, d_lambda, d_evals, mode);
 }
 
+namespace lyap {
+namespace {
+
+/* Shading of stored hit records (kernel.cu:25-106, 511-514): one lane per
+ * pixel, the same shade and RGBA conversion the render applies when a ray
+ * ends, so a frame re-shaded for new lights is byte-identical to one
+ * rendered with them. A miss record (a = NaN) stays transparent black. */
+__global__ __launch_bounds__(256) void shade_points(LyapRGBA *__restrict__ rgba,
+                                                    const LyapPoint *__restrict__ points,
+                                                    LyapCam cam, const LyapLight *lights,
+                                                    uint32_t num_lights, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const LyapPoint p = points[i];
+    const C4 cc = shade(f3(p.P), f3(p.N), p.a, p.c, cam, lights, num_lights);
+    LyapRGBA px;
+    px.r = to_u8(cc.x);
+    px.g = to_u8(cc.y);
+    px.b = to_u8(cc.z);
+    px.a = to_u8(cc.w);
+    rgba[i] = px;
+}
+
+} /* namespace */
+} /* namespace lyap */
+
+extern "C" int lyap_shade_points(lyap_ctx *ctx, LyapRGBA *d_rgba, const LyapPoint *d_points,
+                                 const LyapCam *cam, const LyapLight *d_lights,
+                                 unsigned num_lights, unsigned w, unsigned h)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    if (!cam) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null camera");
+    const uint64_t n = (uint64_t)w * h;
+    if (n == 0) return LYAP_OK;
+    if (!d_rgba || !d_points) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null rgba or points");
+    if (num_lights > LYAP_MAX_LIGHTS) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "more than %d lights", LYAP_MAX_LIGHTS);
+    if (num_lights > 0 && !d_lights) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null lights");
+    if (n > (1ull << 31)) return fail(ctx, LYAP_ERR_UNSUPPORTED, "image larger than 2^31 pixels");
+    hipLaunchKernelGGL(shade_points, dim3((uint32_t)((n + 255u) / 256u)), dim3(256), 0, ctx->stream,
+                       d_rgba, d_points, *cam, d_lights, (uint32_t)num_lights, (uint32_t)n);
+    LYAP_HIP(ctx, hipGetLastError());
+    return LYAP_OK;
+}
+
 extern "C" int lyap_render_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
                                  const LyapRenderMode *mode_in, const LyapTuning *tuning)
 {
