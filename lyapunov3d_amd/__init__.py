@@ -388,7 +388,7 @@ def make_params(**kw) -> LyapParams:
     return prm
 
 
-def volume_mode(precision=PREC_PLAIN, renorm=16, fp64_margin=0.0, converge_tau=0.0,
+def volume_mode(precision=PREC_PLAIN, renorm=0, fp64_margin=0.0, converge_tau=0.0,
                 converge_band=0.0) -> VolumeMode:
     m = VolumeMode()
     m.precision = precision

@@ -19,9 +19,11 @@
  *              r_n u_{n+1}. |u| <= 1 while 0 <= r <= 4, so the running product
  *              M of |u| only shrinks between renormalisations: it is carried
  *              as an FP32 mantissa plus an int32 exponent, renormalised every
- *              block of <= 16 steps, and a block that ends below 2^-100 (where
- *              gradual underflow could have cost bits) is recomputed with a
- *              renormalisation per step. The ln|r_n| terms depend only on the
+ *              block of whole periods (up to 32 steps for the volumes, one
+ *              16- or 32-slot window for the render), and a block that ends
+ *              below the smallest normal float (where gradual underflow could
+ *              have cost bits) is recomputed with a renormalisation per step.
+ *              Renormalising is exact, so the block length changes no bit. The ln|r_n| terms depend only on the
  *              letter, so they are added once per letter with host-computed
  *              counts. One log2 per evaluation instead of one per step.
  *              Per accumulated step: one multiply (u^2), one FMA, one
@@ -51,8 +53,10 @@ namespace lyap {
 
 constexpr float kLn2f = 0.693147180559945309f;
 constexpr double kLn2 = 0.693147180559945309417232121458;
-/* A product block ending below this may have lost bits to underflow. */
-constexpr float kUnderflowGuard = 0x1p-100f;
+/* A product block ending below the smallest normal float may have lost bits
+ * to gradual underflow; one ending at or above it did not, since |u| <= 1
+ * keeps the product non-increasing within the block. */
+constexpr float kUnderflowGuard = 0x1p-126f;
 
 /* Wave-uniform description of one lambda evaluation: iteration counts,
  * sequence, and host-derived loop bounds. Built by make_eval_desc(). */
@@ -125,7 +129,7 @@ inline bool make_eval_desc(EvalDesc *e, const LyapParams &prm,
     e->tail = left % L;
     e->half_periods = e->acc_periods / 2;
     e->half_steps = (e->head_end - e->head_begin) + e->half_periods * (uint32_t)L;
-    if (renorm == 0) renorm = 16;
+    if (renorm == 0) renorm = 32;
     uint32_t bp = renorm / (uint32_t)L;
     e->block_periods = bp ? bp : 1;
     /* letter counts */

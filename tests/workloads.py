@@ -25,12 +25,12 @@ def volume_config(name: str):
     raise KeyError(name)
 
 
-C3_MODE = dict(precision=lyap.PREC_PRODUCT, renorm=16, fp64_margin=0.01)
-C3_STRICT = dict(precision=lyap.PREC_PRODUCT, renorm=16, fp64_margin=0.01,
+C3_MODE = dict(precision=lyap.PREC_PRODUCT, fp64_margin=0.01)
+C3_STRICT = dict(precision=lyap.PREC_PRODUCT, fp64_margin=0.01,
                  converge_tau=1e-4, converge_band=0.1)
 # the same test up to min(half) < 0.2: also re-evaluates weakly chaotic
 # voxels, which FP32 rounding can show as chaotic where FP64 settles
-C3_STRICT_WIDE = dict(precision=lyap.PREC_PRODUCT, renorm=16, fp64_margin=0.01,
+C3_STRICT_WIDE = dict(precision=lyap.PREC_PRODUCT, fp64_margin=0.01,
                       converge_tau=1e-4, converge_band=0.2)
 
 
