@@ -443,7 +443,8 @@ def bench_serving(lyap, timer, ctx, seconds, sat_n):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
+        print(json.dumps({"error": "the benchmark runs on one GPU; volumes and views shard by z slab "
+                                   "or by request across GPUs, which is not run here",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
