@@ -248,10 +248,12 @@ int lyap_shade_points(lyap_ctx *ctx, LyapRGBA *d_rgba, const LyapPoint *d_points
 int lyap_debug_wave_log(lyap_ctx *ctx, void *d_log);
 
 /* Kernel timing: while on, every render kernel launch on the context's
- * stream is bracketed by a pair of HIP events. lyap_timing_collect waits for
- * the stream, returns the summed kernel milliseconds and the number of
- * launches since timing was switched on or last collected, and starts over.
- * No reference counterpart (measurement only). */
+ * stream is bracketed by a pair of HIP events; switching it off pauses
+ * (launches recorded so far stay), so a caller may time a sample of its
+ * frames. lyap_timing_collect waits for the stream, returns the summed
+ * kernel milliseconds and the number of launches recorded since the last
+ * collection, and starts over. No reference counterpart (measurement
+ * only). */
 int lyap_timing_enable(lyap_ctx *ctx, int on);
 int lyap_timing_collect(lyap_ctx *ctx, double *kernel_ms, uint64_t *launches);
 

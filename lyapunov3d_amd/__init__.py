@@ -504,11 +504,11 @@ class Context:
         _check(rc, self.h, "lyap_shade_points")
 
     def timing(self, on: bool):
-        """Brackets every render kernel launch with HIP events while on."""
+        """Brackets every render kernel launch with HIP events while on; off pauses."""
         _check(self.lib.lyap_timing_enable(self.h, 1 if on else 0), self.h, "lyap_timing_enable")
 
     def timing_collect(self):
-        """(summed kernel ms, launches) since timing was switched on or last collected."""
+        """(summed kernel ms, launches) recorded since the last collection."""
         ms = C.c_double()
         n = C.c_uint64()
         _check(self.lib.lyap_timing_collect(self.h, C.byref(ms), C.byref(n)), self.h, "lyap_timing_collect")

@@ -271,8 +271,9 @@ int lyap_last_reevaluated(lyap_ctx *ctx, uint64_t *count)
 int lyap_timing_enable(lyap_ctx *ctx, int on)
 {
     if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    /* switching off pauses: the launches recorded so far stay until
+     * collected, so a caller can time a sample of its frames */
     ctx->timing = on != 0;
-    ctx->timing_used = 0;
     return LYAP_OK;
 }
 

@@ -1348,8 +1348,13 @@ constexpr uint32_t kCounterWords = kHeadStride + 4u * kTileHeads * kHeadStride;
 struct RenderResources {
     void *view_table = nullptr;      /* device ViewDesc[] */
     size_t view_table_bytes = 0;
-    void *staging = nullptr;         /* pinned host copy of the table */
-    size_t staging_bytes = 0;
+    /* pinned host copies of the table, used in turn so a frame's host work
+     * overlaps the previous frame on the device; staged[k] is recorded after
+     * the launches that read staging[k] */
+    void *staging[2] = {nullptr, nullptr};
+    size_t staging_bytes[2] = {0, 0};
+    hipEvent_t staged[2] = {nullptr, nullptr};
+    uint32_t stage = 0;              /* staging buffer of the latest launch */
     uint32_t *counters = nullptr;    /* kCounterWords: [2..3] evaluations, then tile heads */
     void *cont_buf = nullptr;        /* rays handed between levels */
     size_t cont_buf_bytes = 0;
@@ -1376,7 +1381,10 @@ static void release_resources(lyap_ctx *ctx, RenderResources &r)
     ctx_free(ctx, r.cont_buf);
     ctx_free(ctx, r.cont_counts);
     ctx_free(ctx, r.counters);
-    if (r.staging) (void)hipHostFree(r.staging);
+    for (int k = 0; k < 2; ++k) {
+        if (r.staging[k]) (void)hipHostFree(r.staging[k]);
+        if (r.staged[k]) (void)hipEventDestroy(r.staged[k]);
+    }
     r = RenderResources{};
 }
 
@@ -1417,17 +1425,21 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
         if (rc) return rc;
         res.counters = (uint32_t *)c;
     }
-    if (table > res.staging_bytes) {
+    res.stage ^= 1u;
+    const uint32_t k = res.stage;
+    if (table > res.staging_bytes[k]) {
         if (may_sync) LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        if (res.staging) (void)hipHostFree(res.staging);
-        res.staging = nullptr;
-        res.staging_bytes = 0;
-        LYAP_HIP(ctx, hipHostMalloc(&res.staging, table, hipHostMallocDefault));
-        res.staging_bytes = table;
+        if (res.staging[k]) (void)hipHostFree(res.staging[k]);
+        res.staging[k] = nullptr;
+        res.staging_bytes[k] = 0;
+        LYAP_HIP(ctx, hipHostMalloc(&res.staging[k], table, hipHostMallocDefault));
+        res.staging_bytes[k] = table;
     }
-    /* the previous launch's copy out of the staging buffer must be done */
-    if (may_sync) LYAP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    std::memcpy(res.staging, views, table);
+    if (!res.staged[k]) LYAP_HIP(ctx, hipEventCreateWithFlags(&res.staged[k], hipEventDisableTiming));
+    /* the launch before last, which read this buffer, must have copied it
+     * (an event never recorded is complete) */
+    if (may_sync) LYAP_HIP(ctx, hipEventSynchronize(res.staged[k]));
+    std::memcpy(res.staging[k], views, table);
     return LYAP_OK;
 }
 
@@ -1437,7 +1449,7 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
                          unsigned n_views, uint32_t total_tiles, const LaunchPlan &plan,
                          uint32_t W, int precision)
 {
-    LYAP_HIP(ctx, hipMemcpyAsync(res.view_table, res.staging, sizeof(ViewDesc) * n_views,
+    LYAP_HIP(ctx, hipMemcpyAsync(res.view_table, res.staging[res.stage], sizeof(ViewDesc) * n_views,
                                  hipMemcpyHostToDevice, ctx->stream));
     int rc = launch_zero_u32(ctx, res.counters, kCounterWords);
     if (rc) return rc;
@@ -1597,7 +1609,10 @@ static int render_views(lyap_ctx *ctx, std::vector<ViewDesc> &views, uint32_t W,
     uint32_t total_tiles = 0;
     int rc = prepare_views(ctx, res, views.data(), (unsigned)views.size(), plan, W, true, &total_tiles);
     if (rc) return rc;
-    return enqueue_views(ctx, res, views.data(), (unsigned)views.size(), total_tiles, plan, W, mode.precision);
+    rc = enqueue_views(ctx, res, views.data(), (unsigned)views.size(), total_tiles, plan, W, mode.precision);
+    if (rc) return rc;
+    LYAP_HIP(ctx, hipEventRecord(res.staged[res.stage], ctx->stream));
+    return LYAP_OK;
 }
 
 static LyapTuning merged_tuning(const lyap_ctx *ctx, const LyapTuning *t)

@@ -101,6 +101,18 @@ def test_timing_counts_the_level_launches(ctx):
         assert launches2 == 0 and ms2 == 0.0
     finally:
         ctx.timing(False)
+    # switching off pauses: a sample of frames can be timed
+    ctx.timing_collect()
+    try:
+        for on in (True, False, True):
+            ctx.timing(on)
+            bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
+        ctx.timing(False)
+        bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
+        ctx.sync()
+        assert ctx.timing_collect()[1] == 2 * 4
+    finally:
+        ctx.timing(False)
     # direct schedule: one launch per frame
     d = W.c2_mode(fp64_margin=0.0)
     d.schedule = lyap.SCHEDULE_DIRECT
