@@ -6,7 +6,9 @@
    arithmetic);
  - FP64 batches reproduce the oracle exactly for sampled views;
  - a captured frame replayed several times, with other work and
-   synchronisations between replays, matches the eager batch every time.
+   synchronisations between replays, matches the eager batch every time;
+ - render calls issued back to back, without synchronising, keep their own
+   view tables.
 """
 import ctypes as C
 
@@ -78,6 +80,27 @@ def test_batch_equals_single_views(ctx, prec):
             ref = O.render(v["cam"], v["prm"], v["seq"], lights, n_lights, v["w"], v["h"], mode=mode, lam=O.F64)
             assert np.array_equal(got[i]["evals"], ref["evals"])
             assert np.array_equal(got[i]["depth"], ref["depth"], equal_nan=True)
+
+
+def test_back_to_back_calls_keep_their_tables(ctx):
+    """Render calls do not wait for the previous frame: each stages its view
+    table in one of two pinned buffers used in turn. Six different views
+    rendered with no synchronisation between the calls (so the third call
+    reuses the first buffer while frames may still be in flight) give the
+    bytes of rendering each alone."""
+    views, lights, n_lights = small_batch(6, seed=23)
+    ld = device_lights(lights, n_lights)
+    mode = W.c2_mode(fp64_margin=0.0)
+    bufs = alloc(views)
+    for v, b in zip(views, bufs):
+        b.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, mode)
+    ctx.sync()
+    got = [b.fetch() for b in bufs]
+    for v, g in zip(views, got):
+        one = RenderBuffers(v["w"], v["h"])
+        one.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, mode)
+        ctx.sync()
+        same(g, one.fetch())
 
 
 def test_frame_replays_match_eager(ctx):
