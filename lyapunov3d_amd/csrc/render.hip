@@ -237,7 +237,7 @@ struct Ray {
     uint32_t evals;        /* lambda evaluations the method needed */
     uint32_t index;        /* march sample index of (t, P) */
     uint32_t k;            /* refine or normal step counter */
-    uint32_t flags;        /* bit 0: near step, bit 1: walk sign */
+    uint32_t flags;        /* bit 0: near step, bit 1: walk sign, bit 2: hit to shade */
     F3 V, P;
     float t, t1, dt, Fdt, Ndt, last_step;
     float a, c, l;
@@ -359,13 +359,32 @@ __device__ void write_hit(const ViewDesc &v, const Ray &r, F3 N)
     if (o.present & 2u) o.depth[r.pix] = r.t;
     if (o.present & 4u) o.lambda[r.pix] = r.l;
     if (o.present & 8u) o.evals[r.pix] = r.evals;
-    const C4 cc = shade(r.P, N, r.a, r.c, v.cam, v.lights, v.num_lights);
-    LyapRGBA px;
-    px.r = to_u8(cc.x);
-    px.g = to_u8(cc.y);
-    px.b = to_u8(cc.z);
-    px.a = to_u8(cc.w);
-    o.rgba[r.pix] = px;
+}
+
+/* Hits waiting for their colour, per wave, in LDS. Shading a hit as its
+ * ray ends runs the lights loop for one lane of the wave at a time; the
+ * wave instead collects up to 64 hits and shades them with all its lanes at
+ * once (the same shade and RGBA conversion, so the same bytes). */
+struct ShadeList {
+    float px[64], py[64], pz[64], nx[64], ny[64], nz[64], a[64], c[64];
+    uint32_t pix[64];
+};
+
+/* Shades entries [0, n) of the wave's list into view v's RGBA. */
+__device__ void shade_list(const ViewDesc &v, const ShadeList &sl, uint32_t n)
+{
+    const uint32_t i = __lane_id();
+    if (i < n) {
+        const F3 P = {sl.px[i], sl.py[i], sl.pz[i]};
+        const F3 N = {sl.nx[i], sl.ny[i], sl.nz[i]};
+        const C4 cc = shade(P, N, sl.a[i], sl.c[i], v.cam, v.lights, v.num_lights);
+        LyapRGBA px;
+        px.r = to_u8(cc.x);
+        px.g = to_u8(cc.y);
+        px.b = to_u8(cc.z);
+        px.a = to_u8(cc.w);
+        v.rgba[sl.pix[i]] = px;
+    }
 }
 
 /* Starts ray `id` of view v; returns false (after the leader wrote a miss,
@@ -824,6 +843,11 @@ __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStas
         if (ph == PH_NORMAL && r.k >= 6u) {
             const F3 N = normalized({r.gx(), r.gy(), r.gz()});
             if (leader) write_hit(v, r, N);
+            /* the colour is computed later, from the normal kept here */
+            r.gx() = N.x;
+            r.gy() = N.y;
+            r.gz() = N.z;
+            r.flags |= 4u;
             r.phase = PH_IDLE;
             finished = true;
         }
@@ -910,12 +934,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
 {
     __shared__ RoundStash stash[4];   /* each wave's samples of the round */
+    __shared__ ShadeList shades[4];   /* each wave's hits waiting for their colour */
     const uint32_t lane = __lane_id();
     Ray r;
     r.phase = PH_IDLE;
     r.pix = 0;
     r.evals = 0;
     uint32_t done_evals = 0;
+    ShadeList &sh = shades[threadIdx.x >> 6];
+    uint32_t sh_count = 0;            /* entries of sh, all of view cur_view */
 
     /* samples per lane and round; a group of K lanes takes K * S samples */
     const uint32_t S = uniform(a.slots >= 1u && a.slots <= kMaxSlots ? a.slots : 1u);
@@ -990,6 +1017,10 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
                     uint32_t tv, f, l;
                     tile_lookup(views, a, pend_tile, tile, tv, f, l);
                     pend_tile = 0xFFFFFFFFu;
+                    if (sh_count) {
+                        shade_list(views[uniform(cur_view)], sh, sh_count);
+                        sh_count = 0;
+                    }
                     cur_view = tv;
                     pos = f;
                     end = l;
@@ -1152,8 +1183,33 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             c_eval += now - c_mark;
             c_mark = now;
         }
-        if (combine(v, r, evm, st, gbase * S, KS, busy, rank == 0) && rank == 0)
-            done_evals += r.evals;
+        const bool fin = combine(v, r, evm, st, gbase * S, KS, busy, rank == 0) && rank == 0;
+        if (fin) done_evals += r.evals;
+        /* queue this round's hits for shading; shade the queue first when
+         * they would not fit */
+        const uint64_t hm = __ballot(fin && (r.flags & 4u) != 0u);
+        if (hm != 0) {
+            const uint32_t nh = (uint32_t)__popcll(hm);
+            if (sh_count + nh > 64u) {
+                shade_list(v, sh, sh_count);
+                sh_count = 0;
+                wave_sync_lds();
+            }
+            if ((hm >> lane) & 1ull) {
+                const uint32_t i = sh_count + (uint32_t)__popcll(hm & ((1ull << lane) - 1ull));
+                sh.px[i] = r.P.x;
+                sh.py[i] = r.P.y;
+                sh.pz[i] = r.P.z;
+                sh.nx[i] = r.gx();
+                sh.ny[i] = r.gy();
+                sh.nz[i] = r.gz();
+                sh.a[i] = r.a;
+                sh.c[i] = r.c;
+                sh.pix[i] = r.pix;
+            }
+            sh_count += nh;
+            wave_sync_lds();
+        }
         /* hand rays that used this launch's sample budget to the next launch:
          * one atomic per wave reserves the records of all its leaders */
         if (a.budget != 0) {
@@ -1181,6 +1237,7 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             }
         }
     }
+    if (sh_count) shade_list(views[uniform(cur_view)], sh, sh_count);
     /* one 64-bit atomic per wave for the evaluation count */
     unsigned long long sum = done_evals;
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
