@@ -370,6 +370,15 @@ struct ShadeList {
     uint32_t pix[64];
 };
 
+/* Rays of a tile that meet the box, set up by all lanes of the wave at once
+ * (one lane per ray) and then handed to groups as they free up; lanes that
+ * set up a miss write it at once. Setting up each group's ray as it frees
+ * up ran the box clip (six double divisions) for one group at a time. */
+struct SetupList {
+    float vx[64], vy[64], vz[64], t0[64], t1[64];
+    uint32_t pix[64];
+};
+
 /* Shades entries [0, n) of the wave's list into view v's RGBA. */
 __device__ void shade_list(const ViewDesc &v, const ShadeList &sl, uint32_t n)
 {
@@ -943,6 +952,12 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
     uint32_t done_evals = 0;
     ShadeList &sh = shades[threadIdx.x >> 6];
     uint32_t sh_count = 0;            /* entries of sh, all of view cur_view */
+    __shared__ SetupList setups[4];   /* each wave's set-up rays of its tile */
+    SetupList &su = setups[threadIdx.x >> 6];
+    uint32_t hpos = 0, hcount = 0;    /* untaken entries [hpos, hcount) of su */
+    /* pixel rays are set up a tile at a time, except in the reference's
+     * shape, where each lane keeps its own ray of the block */
+    const bool batched = a.level == 0 && !a.direct;
 
     /* samples per lane and round; a group of K lanes takes K * S samples */
     const uint32_t S = uniform(a.slots >= 1u && a.slots <= kMaxSlots ? a.slots : 1u);
@@ -1011,7 +1026,39 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             const bool need = rank == 0 && lane + K <= 64u && r.phase == PH_IDLE && !pend;
             const uint64_t mask = __ballot(need);
             if (mask == 0) break;
-            if (pos >= end) {
+            if (batched && hpos >= hcount && pos < end) {
+                /* set up the tile's next (up to) 64 rays, one per lane */
+                const unsigned long long s0 = a.wave_log ? clock64() : 0ull;
+                const ViewDesc &v = views[uniform(cur_view)];
+                const uint32_t n = min(end - pos, 64u);
+                uint32_t pix = 0;
+                F3 V = {0.f, 0.f, 0.f};
+                float t0 = 0.f, t1 = 0.f;
+                const int k = lane < n ? ray_setup(v, pos + lane, pix, V, t0, t1) : RAY_PAD;
+                if (k == RAY_MISS) {
+                    Ray m;
+                    m.pix = pix;
+                    m.evals = 0;
+                    write_miss(v, m);
+                }
+                const uint64_t hits = __ballot(k == RAY_HIT);
+                if (k == RAY_HIT) {
+                    const uint32_t i = (uint32_t)__popcll(hits & ((1ull << lane) - 1ull));
+                    su.vx[i] = V.x;
+                    su.vy[i] = V.y;
+                    su.vz[i] = V.z;
+                    su.t0[i] = t0;
+                    su.t1[i] = t1;
+                    su.pix[i] = pix;
+                }
+                wave_sync_lds();
+                hpos = 0;
+                hcount = (uint32_t)__popcll(hits);
+                pos += n;
+                if (a.wave_log) c_start += clock64() - s0;
+                continue;
+            }
+            if (pos >= end && hpos >= hcount) {
                 if (pend_tile != 0xFFFFFFFFu) {
                     if (__ballot(r.phase != PH_IDLE || pend) != 0) break;   /* drain first */
                     uint32_t tv, f, l;
@@ -1047,7 +1094,7 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
                 end = l;
                 continue;
             }
-            const uint32_t avail = end - pos;
+            const uint32_t avail = batched ? hcount - hpos : end - pos;
             /* every lane of a needing group takes the group's ray: the
              * group's rank among the needing leaders below it */
             const uint32_t gb = lane - rank;
@@ -1056,7 +1103,14 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             const unsigned long long s0 = a.wave_log ? clock64() : 0ull;
             if (gneed && rk < avail) {
                 const ViewDesc &v = views[uniform(cur_view)];
-                if (a.level == 0) {
+                if (batched) {
+                    const uint32_t i = hpos + rk;
+                    p_pix = su.pix[i];
+                    p_V = {su.vx[i], su.vy[i], su.vz[i]};
+                    p_t0 = su.t0[i];
+                    p_t1 = su.t1[i];
+                    pend = true;
+                } else if (a.level == 0) {
                     const int k = ray_setup(v, pos + rk, p_pix, p_V, p_t0, p_t1);
                     if (k == RAY_HIT) pend = true;
                     else if (k == RAY_MISS && rank == 0) {
@@ -1072,7 +1126,8 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             }
             if (a.wave_log) c_start += clock64() - s0;
             const uint32_t taken = (uint32_t)__popcll(mask);
-            pos += taken < avail ? taken : avail;
+            if (batched) hpos += taken < avail ? taken : avail;
+            else pos += taken < avail ? taken : avail;
         }
         if (pend) {
             if (a.level == 0) start_hit(views[uniform(cur_view)], p_pix, p_V, p_t0, p_t1, r);
@@ -1093,7 +1148,8 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             continue;
         }
         /* ---- no more rays for this view: spread idle lanes over busy rays */
-        const bool starved = !a.direct && pos >= end && (queue_empty || pend_tile != 0xFFFFFFFFu);
+        const bool starved = !a.direct && pos >= end && hpos >= hcount &&
+                             (queue_empty || pend_tile != 0xFFFFFFFFu);
         if (starved) {
             const uint64_t leaders = __ballot(r.phase != PH_IDLE && lane % K == 0u);
             const uint32_t nb = (uint32_t)__popcll(leaders);
