@@ -57,7 +57,7 @@ struct ViewDesc {
     uint32_t cont_cap;
 };
 
-/* March state of a ray handed to the next launch (80 bytes). */
+/* March state of a ray handed to the next launch (96 bytes). */
 struct ContEntry {
     uint32_t pix, evals, index, flags;
     float Vx, Vy, Vz, Px, Py, Pz;
@@ -67,6 +67,7 @@ struct ContEntry {
     uint32_t m;
     uint32_t pad[2];
 };
+static_assert(sizeof(ContEntry) == 96, "continuation record size");
 
 struct RenderArgs {
     const ViewDesc *views;
