@@ -8,7 +8,9 @@
  - a captured frame replayed several times, with other work and
    synchronisations between replays, matches the eager batch every time;
  - render calls issued back to back, without synchronising, keep their own
-   view tables.
+   view tables;
+ - every launch shape (lanes per ray, samples per lane, tile length) gives
+   the same bytes.
 """
 import ctypes as C
 
@@ -143,6 +145,30 @@ def test_autotune_keeps_results(ctx):
     ctx.render_batch(api, mode)
     ctx.sync()
     for b, e in zip(bufs, before):
+        same(b.fetch(), e)
+
+
+@pytest.mark.parametrize("lanes,slots,tile,n", [(1, 1, 128, 3), (1, 1, 200, 3), (4, 1, 32, 3),
+                                                (8, 2, 8, 3), (16, 1, 4, 3), (2, 4, 64, 3),
+                                                (1, 1, 200, 5)])
+def test_launch_shapes_same_bytes(ctx, lanes, slots, tile, n):
+    """Lanes per ray, samples per lane and tile length change only the
+    schedule: a tile longer than a wave is set up in chunks of 64 rays, a
+    short one leaves groups to take rays from the next claim. Five views
+    include the 20-letter sequence (the 32-slot window)."""
+    views, lights, n_lights = small_batch(n=n, seed=11)
+    ld = device_lights(lights, n_lights)
+    mode = W.c2_mode(fp64_margin=0.0)
+    bufs = alloc(views)
+    api = as_api(views, bufs, ld, n_lights)
+    ctx.render_batch(api, mode)
+    ctx.sync()
+    ref = [b.fetch() for b in bufs]
+    t = lyap.Tuning()
+    t.lanes_per_ray, t.slots, t.tile_rays = lanes, slots, tile
+    ctx.render_batch(api, mode, t)
+    ctx.sync()
+    for b, e in zip(bufs, ref):
         same(b.fetch(), e)
 
 
