@@ -1179,6 +1179,12 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             c_mark = now;
         }
         /* ---- S lambda evaluations per lane ------------------------------ */
+        /* The round's control (combine, hand-over, refill) is short, branchy
+         * and waits on LDS and scalar results; the wave runs it at a raised
+         * issue priority and its evaluations at the normal one, so the
+         * SIMD serves the control first and fills the gaps with the other
+         * waves' evaluations (4 % on the 512x512 frame). */
+        __builtin_amdgcn_s_setprio(0);
         const ViewDesc &v = views[uniform(cur_view)];
         const uint32_t rank = lane % K;
         const uint32_t gbase = lane - rank;
@@ -1235,6 +1241,7 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             }
         }
         wave_sync_lds();
+        __builtin_amdgcn_s_setprio(2);
         if (a.wave_log) {
             const unsigned long long now = clock64();
             c_eval += now - c_mark;
