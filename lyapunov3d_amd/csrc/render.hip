@@ -55,6 +55,8 @@ struct ViewDesc {
     uint32_t ray_begin;        /* first ray of this launch (a view may be split) */
     uint32_t cont_base;        /* this view's region of the continuation buffers */
     uint32_t cont_cap;
+    uint32_t refine_base;      /* this view's region of the refinement records */
+    uint32_t refine_cap;       /* records per sub-region (kRefineSubs of them) */
 };
 
 /* March state of a ray handed to the next launch (96 bytes). */
@@ -68,6 +70,23 @@ struct ContEntry {
     uint32_t pad[2];
 };
 static_assert(sizeof(ContEntry) == 96, "continuation record size");
+
+/* A ray whose march found its opaque sample, waiting for the refinement
+ * launch: the bisection interval [lo, hi] and lambda at hi, the march step
+ * (the normal offset of the reference's method), the march sums and the
+ * evaluations so far (48 bytes). */
+struct RefineRec {
+    uint32_t pix, evals;
+    float Vx, Vy, Vz, lo, hi, l_hi, dt, a, c;
+    uint32_t pad;
+};
+static_assert(sizeof(RefineRec) == 48, "refinement record size");
+
+/* Refinement records of a view are kept in kRefineSubs sub-regions, one per
+ * block index modulo kRefineSubs, each with its own counter on a 128-byte
+ * line: waves of one XCD share a counter, and device-scope atomics on a line
+ * that waves of several XCDs update are slow. */
+constexpr uint32_t kRefineSubs = 8;
 
 struct RenderArgs {
     const ViewDesc *views;
@@ -92,6 +111,10 @@ struct RenderArgs {
     const uint32_t *in_counts;         /* per view */
     ContEntry *cont_out;
     uint32_t *out_counts;              /* per view */
+    /* bisection rays park here when their march ends (0: refine in place);
+     * counters per view and sub-region, 32 words apart */
+    RefineRec *refine_out;
+    uint32_t *refine_counts;
 };
 
 namespace {
@@ -1247,8 +1270,47 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
             c_eval += now - c_mark;
             c_mark = now;
         }
+        const int ph0 = r.phase;
         const bool fin = combine(v, r, evm, st, gbase * S, KS, busy, rank == 0) && rank == 0;
         if (fin) done_evals += r.evals;
+        /* a ray whose march ended this round leaves for the refinement
+         * launch, which bisects it on one lane: here a bisection trip would
+         * use K lanes to resolve log2(K+1) levels */
+        if (a.refine_out != nullptr) {
+            const bool pk = busy && ph0 == PH_MARCH && r.phase == PH_REFINE;
+            const uint64_t pm = __ballot(pk && rank == 0);
+            if (pm != 0) {
+                const uint32_t sub = blockIdx.x % kRefineSubs;
+                uint32_t base = 0;
+                if (lane == 0)
+                    base = atomicAdd(a.refine_counts + (uniform(cur_view) * kRefineSubs + sub) * kHeadStride,
+                                     (uint32_t)__popcll(pm));
+                base = uniform((uint32_t)__shfl((int)base, 0));
+                uint32_t ok = 0;
+                if (pk && rank == 0) {
+                    const uint32_t slot = base + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+                    if (slot < v.refine_cap) {
+                        RefineRec q;
+                        q.pix = r.pix;
+                        q.evals = r.evals;
+                        q.Vx = r.V.x;
+                        q.Vy = r.V.y;
+                        q.Vz = r.V.z;
+                        q.lo = r.lo();
+                        q.hi = r.hi();
+                        q.l_hi = r.l_hi();
+                        q.dt = r.dt;
+                        q.a = r.a;
+                        q.c = r.c;
+                        q.pad = 0;
+                        a.refine_out[v.refine_base + sub * v.refine_cap + slot] = q;
+                        ok = 1;
+                    }
+                }
+                ok = (uint32_t)__shfl((int)ok, (int)gbase);
+                if (pk && ok) r.phase = PH_IDLE;
+            }
+        }
         /* queue this round's hits for shading; shade the queue first when
          * they would not fit */
         const uint64_t hm = __ballot(fin && (r.flags & 4u) != 0u);
@@ -1320,6 +1382,104 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
         o[9] = c_start;
         for (uint32_t k = 10; k < kWaveLogWords; ++k) o[k] = 0;
     }
+}
+
+
+/* ---- refinement launch -------------------------------------------------- */
+
+struct RefineArgs {
+    uint32_t n_views;
+    const RefineRec *recs;
+    const uint32_t *counts;            /* per view and sub-region, kHeadStride words apart */
+    unsigned long long *eval_counter;
+};
+
+/* Records of view-sub-region pair p that the march launches wrote. */
+__device__ __forceinline__ uint32_t refine_count(const ViewDesc *__restrict__ views,
+                                                 const RefineArgs &a, uint32_t p)
+{
+    return uniform(min(a.counts[p * kHeadStride], views[uniform(p / kRefineSubs)].refine_cap));
+}
+
+/* Finishes the parked rays, one lane per ray and 64 rays of one view per
+ * wave: the bisection of [lo, hi] (kernel.cu:402-441 with the midpoint
+ * rule), the six normal samples in the order x-, x+, y-, y+, z-, z+
+ * (kernel.cu:456-473), the hit record and its colour. The arithmetic is the
+ * march kernel's, step for step, so the bytes are those of refining in
+ * place. */
+template <int W, int PREC, bool MARGIN>
+__global__ __launch_bounds__(256) void refine_rays(const ViewDesc *__restrict__ views, RefineArgs a)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t n_waves = gridDim.x * 4u;
+    const uint32_t wid = uniform(blockIdx.x * 4u + threadIdx.x / 64u);
+    const uint32_t pairs = a.n_views * kRefineSubs;
+    uint32_t total = 0;
+    for (uint32_t p = 0; p < pairs; ++p) total += (refine_count(views, a, p) + 63u) / 64u;
+    unsigned long long done = 0;
+    for (uint32_t t = wid; t < total; t += n_waves) {
+        uint32_t acc = 0, p = 0, n = 0;
+        for (; p < pairs; ++p) {
+            n = refine_count(views, a, p);
+            const uint32_t nt = (n + 63u) / 64u;
+            if (t < acc + nt) break;
+            acc += nt;
+        }
+        const ViewDesc &v = views[uniform(p / kRefineSubs)];
+        const uint32_t i = (t - acc) * 64u + lane;
+        if (i < n) {
+            const RefineRec q = a.recs[v.refine_base + (p % kRefineSubs) * v.refine_cap + i];
+            const F3 C = f3(v.cam.C);
+            const F3 V = {q.Vx, q.Vy, q.Vz};
+            const float thr = v.prm.opaqueThreshold;
+            float lo = q.lo, hi = q.hi, l_hi = q.l_hi;
+            uint32_t evals = q.evals;
+            for (int k = 0; k < v.bisect_steps; ++k) {
+                const float mid = lo + (hi - lo) * 0.5f;
+                const F3 sp = add(C, mul(V, mid));
+                float lam = eval_lambda<W, PREC>(v.e, sp.x, sp.y, sp.z);
+                if (MARGIN && PREC != LYAP_PREC_FP64 && !(__builtin_fabsf(lam - thr) >= v.margin))
+                    lam = eval_fp64_call<W>(v.e, sp.x, sp.y, sp.z);
+                ++evals;
+                if (lam > thr) lo = mid;
+                else { hi = mid; l_hi = lam; }
+            }
+            Ray r;
+            r.pix = q.pix;
+            r.t = hi;
+            r.P = add(C, mul(V, hi));
+            r.l = l_hi;
+            r.a = q.a;
+            r.c = q.c;
+            const float h = (v.normal_h > 0.0f) ? v.normal_h : q.dt * v.prm.gradient;
+            float g[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (uint32_t k = 0; k < 6u; ++k) {
+                F3 sp = r.P;
+                const float off = (k & 1u) ? h : -h;
+                if (k < 2) sp.x = r.P.x + off;
+                else if (k < 4) sp.y = r.P.y + off;
+                else sp.z = r.P.z + off;
+                const float lam = eval_lambda<W, PREC>(v.e, sp.x, sp.y, sp.z);
+                ++evals;
+                const float sl = (k & 1u) ? lam : -lam;
+                g[k / 2u] = (k % 2u == 0u) ? sl : g[k / 2u] + sl;
+            }
+            r.evals = evals;
+            const F3 N = normalized({g[0], g[1], g[2]});
+            write_hit(v, r, N);
+            const C4 cc = shade(r.P, N, r.a, r.c, v.cam, v.lights, v.num_lights);
+            LyapRGBA px;
+            px.r = to_u8(cc.x);
+            px.g = to_u8(cc.y);
+            px.b = to_u8(cc.z);
+            px.a = to_u8(cc.w);
+            v.rgba[r.pix] = px;
+            done += evals;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) done += __shfl_down(done, off);
+    if (lane == 0 && done) atomicAdd(a.eval_counter, done);
 }
 
 } /* namespace */
@@ -1483,6 +1643,10 @@ struct RenderResources {
     size_t cont_buf_bytes = 0;
     void *cont_counts = nullptr;     /* per view and level */
     size_t cont_counts_bytes = 0;
+    void *refine_buf = nullptr;      /* rays waiting for the refinement launch */
+    size_t refine_buf_bytes = 0;
+    void *refine_counts = nullptr;   /* per view and sub-region, one 128 B line each */
+    size_t refine_counts_bytes = 0;
 };
 
 static int ensure_buffer(lyap_ctx *ctx, void **buf, size_t *cap, size_t bytes, bool may_sync)
@@ -1503,12 +1667,22 @@ static void release_resources(lyap_ctx *ctx, RenderResources &r)
     ctx_free(ctx, r.view_table);
     ctx_free(ctx, r.cont_buf);
     ctx_free(ctx, r.cont_counts);
+    ctx_free(ctx, r.refine_buf);
+    ctx_free(ctx, r.refine_counts);
     ctx_free(ctx, r.counters);
     for (int k = 0; k < 2; ++k) {
         if (r.staging[k]) (void)hipHostFree(r.staging[k]);
         if (r.staged[k]) (void)hipEventDestroy(r.staged[k]);
     }
     r = RenderResources{};
+}
+
+/* Whether the launch parks bisection rays for the refinement launch: the
+ * batched schedule with bisection (the walk refines in place, and the
+ * reference's launch shape keeps every ray on its lane). */
+static bool parks(const LaunchPlan &plan, const ViewDesc *views)
+{
+    return !plan.direct && views[0].refine == LYAP_REFINE_BISECT && views[0].bisect_steps > 0;
 }
 
 /* Prepares `views` for a launch: tile ranges, continuation regions, the
@@ -1519,7 +1693,7 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
                          bool may_sync, uint32_t *total_tiles)
 {
     uint32_t tiles = 0;
-    uint64_t cap_total = 0;
+    uint64_t cap_total = 0, refine_total = 0;
     for (unsigned i = 0; i < n_views; ++i) {
         if (views[i].e.W != W) rebuild_blocks(&views[i].e, W);
         views[i].tile_begin = tiles;
@@ -1531,8 +1705,16 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
         views[i].cont_base = (uint32_t)cap_total;
         views[i].cont_cap = cap;
         cap_total += cap;
+        /* refinement records: each sub-region holds a quarter of the view's
+         * rays, so one XCD may park twice its share before a ray has to be
+         * refined in place */
+        uint32_t rcap = (views[i].n_rays + 3u) / 4u;
+        if (rcap < 64u) rcap = 64u;
+        views[i].refine_base = (uint32_t)refine_total;
+        views[i].refine_cap = rcap;
+        refine_total += (uint64_t)rcap * kRefineSubs;
     }
-    if (cap_total > 0xFFFFFFFFull)
+    if (cap_total > 0xFFFFFFFFull || refine_total > 0xFFFFFFFFull)
         return fail(ctx, LYAP_ERR_UNSUPPORTED, "too many rays in one launch");
     *total_tiles = tiles;
     const size_t table = sizeof(ViewDesc) * n_views;
@@ -1540,6 +1722,14 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
     if (rc) return rc;
     rc = ensure_buffer(ctx, &res.cont_counts, &res.cont_counts_bytes, 3 * n_views * sizeof(uint32_t), may_sync);
     if (rc) return rc;
+    if (parks(plan, views)) {
+        rc = ensure_buffer(ctx, &res.refine_buf, &res.refine_buf_bytes, refine_total * sizeof(RefineRec),
+                           may_sync);
+        if (rc) return rc;
+        rc = ensure_buffer(ctx, &res.refine_counts, &res.refine_counts_bytes,
+                           (size_t)n_views * kRefineSubs * kHeadStride * sizeof(uint32_t), may_sync);
+        if (rc) return rc;
+    }
     rc = ensure_buffer(ctx, &res.view_table, &res.view_table_bytes, table, may_sync);
     if (rc) return rc;
     if (!res.counters) {
@@ -1578,6 +1768,11 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
     if (rc) return rc;
     rc = launch_zero_u32(ctx, (uint32_t *)res.cont_counts, 3 * n_views);
     if (rc) return rc;
+    const bool park = parks(plan, views);
+    if (park) {
+        rc = launch_zero_u32(ctx, (uint32_t *)res.refine_counts, n_views * kRefineSubs * kHeadStride);
+        if (rc) return rc;
+    }
     uint64_t cap_total = 0;
     for (unsigned i = 0; i < n_views; ++i) cap_total += views[i].cont_cap;
     ContEntry *bufA = (ContEntry *)res.cont_buf;
@@ -1610,6 +1805,8 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         a.in_counts = level ? counts + (level - 1) * n_views : nullptr;
         a.cont_out = out_buf[level];
         a.out_counts = level < 3 ? counts + level * n_views : nullptr;
+        a.refine_out = park ? (RefineRec *)res.refine_buf : nullptr;
+        a.refine_counts = park ? (uint32_t *)res.refine_counts : nullptr;
         uint32_t blocks = plan.blocks;
         if (level == 0 && blocks > needed0) blocks = (uint32_t)needed0;
         if (blocks == 0) blocks = 1;
@@ -1619,6 +1816,34 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
 #define LYAP_LAUNCH(WW, PP, MM)                                                       \
     hipLaunchKernelGGL((render_persistent<WW, PP, MM>), dim3(blocks), dim3(256), 0, ctx->stream, \
                        (const ViewDesc *)res.view_table, a)
+        if (W == 16) {
+            if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(16, LYAP_PREC_PLAIN, false);
+            else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, true);
+            else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, false);
+            else LYAP_LAUNCH(16, LYAP_PREC_FP64, false);
+        } else {
+            if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(32, LYAP_PREC_PLAIN, false);
+            else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(32, LYAP_PREC_PRODUCT, true);
+            else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH(32, LYAP_PREC_PRODUCT, false);
+            else LYAP_LAUNCH(32, LYAP_PREC_FP64, false);
+        }
+#undef LYAP_LAUNCH
+        LYAP_HIP(ctx, hipGetLastError());
+        rc = timing_mark(ctx);
+        if (rc) return rc;
+    }
+    if (park) {
+        RefineArgs ra{};
+        ra.n_views = n_views;
+        ra.recs = (const RefineRec *)res.refine_buf;
+        ra.counts = (const uint32_t *)res.refine_counts;
+        ra.eval_counter = (unsigned long long *)(res.counters + 2);
+        const uint32_t blocks = cus * 8u;
+        rc = timing_mark(ctx);
+        if (rc) return rc;
+#define LYAP_LAUNCH(WW, PP, MM)                                                       \
+    hipLaunchKernelGGL((refine_rays<WW, PP, MM>), dim3(blocks), dim3(256), 0, ctx->stream, \
+                       (const ViewDesc *)res.view_table, ra)
         if (W == 16) {
             if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(16, LYAP_PREC_PLAIN, false);
             else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, true);

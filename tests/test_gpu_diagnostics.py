@@ -95,7 +95,8 @@ def test_timing_counts_the_level_launches(ctx):
             bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
         ctx.sync()
         ms, launches = ctx.timing_collect()
-        assert launches == 3 * 4 and ms > 0.0
+        # four march levels and the refinement launch per frame
+        assert launches == 3 * 5 and ms > 0.0
         # collecting resets the sums
         ms2, launches2 = ctx.timing_collect()
         assert launches2 == 0 and ms2 == 0.0
@@ -110,7 +111,7 @@ def test_timing_counts_the_level_launches(ctx):
         ctx.timing(False)
         bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
         ctx.sync()
-        assert ctx.timing_collect()[1] == 2 * 4
+        assert ctx.timing_collect()[1] == 2 * 5
     finally:
         ctx.timing(False)
     # direct schedule: one launch per frame
