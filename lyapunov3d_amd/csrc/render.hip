@@ -72,13 +72,12 @@ struct ContEntry {
 static_assert(sizeof(ContEntry) == 96, "continuation record size");
 
 /* A ray whose march found its opaque sample, waiting for the refinement
- * launch: the bisection interval [lo, hi] and lambda at hi, the march step
- * (the normal offset of the reference's method), the march sums and the
- * evaluations so far (48 bytes). */
+ * launch: the bisection interval [lo, hi] with lambda at both ends, the
+ * march step (the normal offset of the reference's method), the march sums
+ * and the evaluations so far (48 bytes). */
 struct RefineRec {
     uint32_t pix, evals;
-    float Vx, Vy, Vz, lo, hi, l_hi, dt, a, c;
-    uint32_t pad;
+    float Vx, Vy, Vz, lo, hi, l_hi, dt, a, c, l_lo;
 };
 static_assert(sizeof(RefineRec) == 48, "refinement record size");
 
@@ -266,8 +265,9 @@ struct Ray {
     float t, t1, dt, Fdt, Ndt, last_step;
     float a, c, l;
     /* phase storage, one set live at a time: the reversing walk's
-     * {Qdt, Qt0, Qt1, minQdt, QdV} in q[0..6], the bisection's {lo, hi, l_hi}
-     * in q[0..2], the normal's {h, gx, gy, gz} (central differences) in q[3..6] */
+     * {Qdt, Qt0, Qt1, minQdt, QdV} in q[0..6], the bisection's {lo, hi, l_hi,
+     * l_lo} in q[0..3], the normal's {h, gx, gy, gz} (central differences) in
+     * q[3..6] */
     float q[7];
     __device__ float &lo() { return q[0]; }
     __device__ float lo() const { return q[0]; }
@@ -275,6 +275,8 @@ struct Ray {
     __device__ float hi() const { return q[1]; }
     __device__ float &l_hi() { return q[2]; }
     __device__ float l_hi() const { return q[2]; }
+    __device__ float &l_lo() { return q[3]; }
+    __device__ float l_lo() const { return q[3]; }
     __device__ float &Qdt() { return q[0]; }
     __device__ float Qdt() const { return q[0]; }
     __device__ float &Qt0() { return q[1]; }
@@ -543,7 +545,7 @@ __device__ __forceinline__ bool walk_continues(const Ray &r)
            (r.Qdt() <= -r.minQdt() || r.Qdt() >= r.minQdt());
 }
 
-__device__ __forceinline__ void enter_refine(const ViewDesc &v, Ray &r)
+__device__ __forceinline__ void enter_refine(const ViewDesc &v, Ray &r, float l_prev)
 {
     r.phase = PH_REFINE;
     r.k = 0;
@@ -551,6 +553,7 @@ __device__ __forceinline__ void enter_refine(const ViewDesc &v, Ray &r)
         r.lo() = r.t - r.last_step;
         r.hi() = r.t;
         r.l_hi() = r.l;
+        r.l_lo() = l_prev;
         if (v.bisect_steps <= 0) enter_normal(v, r);
     } else {
         r.flags &= ~2u;
@@ -647,6 +650,7 @@ __device__ __forceinline__ int march_consume(const ViewDesc &v, Ray &r, float l,
     const LyapParams &prm = v.prm;
     ++r.evals;
     ++r.launch_samples;
+    const float l_prev = r.l;
     r.l = l;
     bool toggled = false;
     if (r.index > 0) {
@@ -683,7 +687,7 @@ __device__ __forceinline__ int march_consume(const ViewDesc &v, Ray &r, float l,
         /* under jitter only the first lane evaluated (its step needs lambda) */
         return (toggled || prm.jitter != 0.0f) ? 2 : 0;
     }
-    enter_refine(v, r);
+    enter_refine(v, r, l_prev);
     return 1;
 }
 
@@ -1302,7 +1306,7 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
                         q.dt = r.dt;
                         q.a = r.a;
                         q.c = r.c;
-                        q.pad = 0;
+                        q.l_lo = r.l_lo();
                         a.refine_out[v.refine_base + sub * v.refine_cap + slot] = q;
                         ok = 1;
                     }
@@ -1401,71 +1405,108 @@ __device__ __forceinline__ uint32_t refine_count(const ViewDesc *__restrict__ vi
     return uniform(min(a.counts[p * kHeadStride], views[uniform(p / kRefineSubs)].refine_cap));
 }
 
-/* Finishes the parked rays, one lane per ray and 64 rays of one view per
- * wave: the bisection of [lo, hi] (kernel.cu:402-441 with the midpoint
- * rule), the six normal samples in the order x-, x+, y-, y+, z-, z+
- * (kernel.cu:456-473), the hit record and its colour. The arithmetic is the
- * march kernel's, step for step, so the bytes are those of refining in
- * place. */
+/* Finishes the parked rays, two lanes per ray and 32 rays of one view per
+ * wave: the bisection of [lo, hi] (the midpoint rule), the six normal
+ * samples in the order x-, x+, y-, y+, z-, z+ (kernel.cu:456-473), the hit
+ * record and its colour. Per bisection trip the first lane evaluates the
+ * midpoint and the second the midpoint of the half the crossing of the
+ * line through lambda at lo and hi falls in; when that half is the one the
+ * first sample chooses, the trip resolves two levels. The normal takes three
+ * trips, the minus samples on the first lane and the plus samples on the
+ * second. The arithmetic is the march kernel's, step for step, so the bytes
+ * are those of refining in place. */
+template <int W, int PREC, bool MARGIN>
+__device__ __forceinline__ float refine_eval(const ViewDesc &v, F3 sp, float thr)
+{
+    float lam = eval_lambda<W, PREC>(v.e, sp.x, sp.y, sp.z);
+    if (MARGIN && PREC != LYAP_PREC_FP64 && !(__builtin_fabsf(lam - thr) >= v.margin))
+        lam = eval_fp64_call<W>(v.e, sp.x, sp.y, sp.z);
+    return lam;
+}
+
 template <int W, int PREC, bool MARGIN>
 __global__ __launch_bounds__(256) void refine_rays(const ViewDesc *__restrict__ views, RefineArgs a)
 {
     const uint32_t lane = __lane_id();
+    const uint32_t side = lane & 1u;           /* 0: midpoints, minus normal samples */
     const uint32_t n_waves = gridDim.x * 4u;
     const uint32_t wid = uniform(blockIdx.x * 4u + threadIdx.x / 64u);
     const uint32_t pairs = a.n_views * kRefineSubs;
     uint32_t total = 0;
-    for (uint32_t p = 0; p < pairs; ++p) total += (refine_count(views, a, p) + 63u) / 64u;
+    for (uint32_t p = 0; p < pairs; ++p) total += (refine_count(views, a, p) + 31u) / 32u;
     unsigned long long done = 0;
     for (uint32_t t = wid; t < total; t += n_waves) {
         uint32_t acc = 0, p = 0, n = 0;
         for (; p < pairs; ++p) {
             n = refine_count(views, a, p);
-            const uint32_t nt = (n + 63u) / 64u;
+            const uint32_t nt = (n + 31u) / 32u;
             if (t < acc + nt) break;
             acc += nt;
         }
         const ViewDesc &v = views[uniform(p / kRefineSubs)];
-        const uint32_t i = (t - acc) * 64u + lane;
-        if (i < n) {
-            const RefineRec q = a.recs[v.refine_base + (p % kRefineSubs) * v.refine_cap + i];
-            const F3 C = f3(v.cam.C);
-            const F3 V = {q.Vx, q.Vy, q.Vz};
-            const float thr = v.prm.opaqueThreshold;
-            float lo = q.lo, hi = q.hi, l_hi = q.l_hi;
-            uint32_t evals = q.evals;
-            for (int k = 0; k < v.bisect_steps; ++k) {
-                const float mid = lo + (hi - lo) * 0.5f;
-                const F3 sp = add(C, mul(V, mid));
-                float lam = eval_lambda<W, PREC>(v.e, sp.x, sp.y, sp.z);
-                if (MARGIN && PREC != LYAP_PREC_FP64 && !(__builtin_fabsf(lam - thr) >= v.margin))
-                    lam = eval_fp64_call<W>(v.e, sp.x, sp.y, sp.z);
+        const uint32_t i = (t - acc) * 32u + (lane >> 1);
+        const bool has = i < n;
+        RefineRec q;
+        if (has) q = a.recs[v.refine_base + (p % kRefineSubs) * v.refine_cap + i];
+        const F3 C = f3(v.cam.C);
+        const F3 V = {q.Vx, q.Vy, q.Vz};
+        const float thr = v.prm.opaqueThreshold;
+        float lo = q.lo, hi = q.hi, l_hi = q.l_hi, l_lo = q.l_lo;
+        uint32_t evals = q.evals;
+        /* both lanes of a ray hold the same interval; trips are wave-uniform */
+        for (int k = 0; __ballot(has && k < v.bisect_steps) != 0;) {
+            const bool go = has && k < v.bisect_steps;
+            const float mid = lo + (hi - lo) * 0.5f;
+            /* the half the line's crossing falls in, and its midpoint */
+            const float f = (l_lo - thr) / (l_lo - l_hi);
+            const bool guess_lo = !(lo + (hi - lo) * f > mid);
+            const float mid2 = guess_lo ? lo + (mid - lo) * 0.5f : mid + (hi - mid) * 0.5f;
+            float lam = 0.f;
+            if (go) lam = refine_eval<W, PREC, MARGIN>(v, add(C, mul(V, side ? mid2 : mid)), thr);
+            const float lm = __shfl(lam, (int)(lane & ~1u));
+            const float lg = __shfl(lam, (int)(lane | 1u));
+            if (go) {
                 ++evals;
-                if (lam > thr) lo = mid;
-                else { hi = mid; l_hi = lam; }
+                ++k;
+                const bool opaque = !(lm > thr);
+                if (opaque) { hi = mid; l_hi = lm; }
+                else { lo = mid; l_lo = lm; }
+                if (k < v.bisect_steps && opaque == guess_lo) {
+                    /* the guessed half is the one taken: its midpoint, formed
+                     * from the same interval, is the next level's */
+                    ++evals;
+                    ++k;
+                    const float m2 = lo + (hi - lo) * 0.5f;
+                    if (!(lg > thr)) { hi = m2; l_hi = lg; }
+                    else { lo = m2; l_lo = lg; }
+                }
             }
-            Ray r;
-            r.pix = q.pix;
-            r.t = hi;
-            r.P = add(C, mul(V, hi));
-            r.l = l_hi;
-            r.a = q.a;
-            r.c = q.c;
-            const float h = (v.normal_h > 0.0f) ? v.normal_h : q.dt * v.prm.gradient;
-            float g[3] = {0.f, 0.f, 0.f};
+        }
+        Ray r;
+        r.pix = q.pix;
+        r.t = hi;
+        r.P = add(C, mul(V, hi));
+        r.l = l_hi;
+        r.a = q.a;
+        r.c = q.c;
+        const float h = (v.normal_h > 0.0f) ? v.normal_h : q.dt * v.prm.gradient;
+        float g[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-            for (uint32_t k = 0; k < 6u; ++k) {
-                F3 sp = r.P;
-                const float off = (k & 1u) ? h : -h;
-                if (k < 2) sp.x = r.P.x + off;
-                else if (k < 4) sp.y = r.P.y + off;
-                else sp.z = r.P.z + off;
-                const float lam = eval_lambda<W, PREC>(v.e, sp.x, sp.y, sp.z);
-                ++evals;
-                const float sl = (k & 1u) ? lam : -lam;
-                g[k / 2u] = (k % 2u == 0u) ? sl : g[k / 2u] + sl;
-            }
-            r.evals = evals;
+        for (uint32_t ax = 0; ax < 3u; ++ax) {
+            F3 sp = r.P;
+            const float off = side ? h : -h;
+            if (ax == 0) sp.x = r.P.x + off;
+            else if (ax == 1) sp.y = r.P.y + off;
+            else sp.z = r.P.z + off;
+            float lam = 0.f;
+            if (has) lam = eval_lambda<W, PREC>(v.e, sp.x, sp.y, sp.z);
+            const float lminus = __shfl(lam, (int)(lane & ~1u));
+            const float lplus = __shfl(lam, (int)(lane | 1u));
+            g[ax] = -lminus;
+            g[ax] = g[ax] + lplus;
+        }
+        if (has && side == 0) {
+            r.evals = evals + 6u;
             const F3 N = normalized({g[0], g[1], g[2]});
             write_hit(v, r, N);
             const C4 cc = shade(r.P, N, r.a, r.c, v.cam, v.lights, v.num_lights);
@@ -1475,13 +1516,12 @@ __global__ __launch_bounds__(256) void refine_rays(const ViewDesc *__restrict__ 
             px.b = to_u8(cc.z);
             px.a = to_u8(cc.w);
             v.rgba[r.pix] = px;
-            done += evals;
+            done += evals + 6u;
         }
     }
     for (int off = 32; off > 0; off >>= 1) done += __shfl_down(done, off);
     if (lane == 0 && done) atomicAdd(a.eval_counter, done);
 }
-
 } /* namespace */
 
 /* Fills a ViewDesc from the API arguments. */
