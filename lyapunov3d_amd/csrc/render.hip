@@ -1278,9 +1278,10 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
         const bool fin = combine(v, r, evm, st, gbase * S, KS, busy, rank == 0) && rank == 0;
         if (fin) done_evals += r.evals;
         /* a ray whose march ended this round leaves for the refinement
-         * launch, which bisects it on one lane: here a bisection trip would
-         * use K lanes to resolve log2(K+1) levels */
-        if (a.refine_out != nullptr) {
+         * launch, which bisects it on two lanes: here a bisection trip would
+         * use K lanes to resolve log2(K+1) levels. A wave at one lane per ray
+         * refines in place, as efficiently and without the record. */
+        if (a.refine_out != nullptr && K >= 2u) {
             const bool pk = busy && ph0 == PH_MARCH && r.phase == PH_REFINE;
             const uint64_t pm = __ballot(pk && rank == 0);
             if (pm != 0) {
