@@ -509,14 +509,14 @@ def main():
 
     rays = w * h
     frame_ms = float(per.mean())
-    # the render kernel: four level launches per frame; algorithmic FLOP of
-    # a frame over the summed duration of its launches (HIP events on the
-    # library stream around each launch)
+    # the render kernels: four march level launches and the refinement
+    # launch per frame; algorithmic FLOP of a frame over the summed duration
+    # of its launches (HIP events on the library stream around each launch)
     launch_ms = kernel_ms / max(launches, 1)
     timed_frames = len(range(0, args.steps, 4))
     launches_per_frame = launches / timed_frames
     achieved = evals_per_frame * flops(prm) / launches_per_frame / (launch_ms * 1e-3) / 1e12
-    traffic = profiled_traffic("render_persistent")
+    traffic = profiled_traffic("render_frame")
     line = {
         "metric": "lambda-evals/s & rays/s on 1 MI355X; p50/p99 latency, served vs plain",
         "value": rays / (ms * 1e-3),
@@ -548,13 +548,14 @@ def main():
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": traffic,
             "flop_per_lambda_eval": flops(prm),
-            "kernel": "render_persistent",
+            "kernel": "render_persistent (4 levels) + refine_rays",
             "launches_per_frame": launches_per_frame,
             "avg_launch_us": launch_ms * 1e3,
             "flop_per_launch": evals_per_frame * flops(prm) / launches_per_frame,
             "note": "achieved = algorithmic FLOP per launch (lambda evaluations x (3 settle + 6 accum)) / "
-                    "mean launch duration from HIP events on the library stream; traffic = HBM bytes per "
-                    "launch from rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy",
+                    "mean launch duration from HIP events on the library stream, over the frame's five "
+                    "launches; traffic = HBM bytes per launch (mean of the five) from rocprofv3 "
+                    "FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy",
         },
     }
     which = [x for x in args.workloads.split(",") if x]
