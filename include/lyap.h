@@ -203,7 +203,9 @@ typedef struct LyapRenderMode {
 } LyapRenderMode;
 
 /* How rays are spread over lanes. BATCHED: the persistent work queue with
- * lane groups per ray, refill of finished lanes and continuation launches.
+ * lane groups per ray, refill of finished lanes and continuation launches;
+ * with bisection, rays whose march found its opaque sample are refined in a
+ * launch of their own, two lanes per ray.
  * DIRECT: the reference's own launch shape (kernel.cu:500-516): one lane
  * per ray for 64 consecutive rays (an 8x8 pixel block), each lane marching
  * its ray to the end and a wave taking new rays only when all 64 are done.
