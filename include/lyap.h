@@ -302,8 +302,9 @@ int lyap_render_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
 int lyap_autotune(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
                   const LyapRenderMode *mode, LyapTuning *best);
 
-/* A captured frame: the batch upload and the persistent launch recorded once
- * as a HIP graph and replayed. The views' device buffers must stay valid. */
+/* A captured frame: the batch upload and the frame's launches (the march
+ * levels and the refinement) recorded once as a HIP graph and replayed. The
+ * views' device buffers must stay valid. */
 typedef struct lyap_frame lyap_frame;
 int lyap_frame_capture(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
                        const LyapRenderMode *mode, const LyapTuning *tuning,
