@@ -1523,6 +1523,19 @@ __global__ __launch_bounds__(256) void refine_rays(const ViewDesc *__restrict__ 
     for (int off = 32; off > 0; off >>= 1) done += __shfl_down(done, off);
     if (lane == 0 && done) atomicAdd(a.eval_counter, done);
 }
+/* Up to three word arrays cleared by one launch: blockIdx.y picks the array. */
+struct ZeroRegions {
+    uint32_t *p[3];
+    uint32_t n[3];
+};
+
+__global__ void zero_regions(ZeroRegions z)
+{
+    const uint32_t r = blockIdx.y;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (z.p[r] != nullptr && i < z.n[r]) z.p[r][i] = 0u;
+}
+
 } /* namespace */
 
 /* Fills a ViewDesc from the API arguments. */
@@ -1805,15 +1818,20 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
 {
     LYAP_HIP(ctx, hipMemcpyAsync(res.view_table, res.staging[res.stage], sizeof(ViewDesc) * n_views,
                                  hipMemcpyHostToDevice, ctx->stream));
-    int rc = launch_zero_u32(ctx, res.counters, kCounterWords);
-    if (rc) return rc;
-    rc = launch_zero_u32(ctx, (uint32_t *)res.cont_counts, 3 * n_views);
-    if (rc) return rc;
+    /* the frame's counters, cleared by one kernel (a graph may hold no
+     * memset nodes) */
     const bool park = parks(plan, views);
-    if (park) {
-        rc = launch_zero_u32(ctx, (uint32_t *)res.refine_counts, n_views * kRefineSubs * kHeadStride);
-        if (rc) return rc;
-    }
+    ZeroRegions z{};
+    z.p[0] = res.counters;
+    z.n[0] = kCounterWords;
+    z.p[1] = (uint32_t *)res.cont_counts;
+    z.n[1] = 3 * n_views;
+    z.p[2] = park ? (uint32_t *)res.refine_counts : nullptr;
+    z.n[2] = park ? n_views * kRefineSubs * kHeadStride : 0u;
+    const uint32_t zmax = std::max(z.n[0], std::max(z.n[1], z.n[2]));
+    hipLaunchKernelGGL(zero_regions, dim3((zmax + 255u) / 256u, 3), dim3(256), 0, ctx->stream, z);
+    LYAP_HIP(ctx, hipGetLastError());
+    int rc = LYAP_OK;
     uint64_t cap_total = 0;
     for (unsigned i = 0; i < n_views; ++i) cap_total += views[i].cont_cap;
     ContEntry *bufA = (ContEntry *)res.cont_buf;
