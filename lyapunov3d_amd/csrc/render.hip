@@ -1447,7 +1447,7 @@ __global__ __launch_bounds__(256) void refine_rays(const ViewDesc *__restrict__ 
         const ViewDesc &v = views[uniform(p / kRefineSubs)];
         const uint32_t i = (t - acc) * 32u + (lane >> 1);
         const bool has = i < n;
-        RefineRec q;
+        RefineRec q{};
         if (has) q = a.recs[v.refine_base + (p % kRefineSubs) * v.refine_cap + i];
         const F3 C = f3(v.cam.C);
         const F3 V = {q.Vx, q.Vy, q.Vz};
@@ -1919,7 +1919,6 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         rc = timing_mark(ctx);
         if (rc) return rc;
     }
-    ctx->last_counters = res.counters;
     return LYAP_OK;
 }
 
@@ -2018,6 +2017,9 @@ static int render_views(lyap_ctx *ctx, std::vector<ViewDesc> &views, uint32_t W,
     if (rc) return rc;
     rc = enqueue_views(ctx, res, views.data(), (unsigned)views.size(), total_tiles, plan, W, mode.precision);
     if (rc) return rc;
+    /* lyap_last_evals reads this frame's counters (a capture, which
+     * enqueues without running, leaves the previous call's in place) */
+    ctx->last_counters = res.counters;
     LYAP_HIP(ctx, hipEventRecord(res.staged[res.stage], ctx->stream));
     return LYAP_OK;
 }
@@ -2058,6 +2060,7 @@ RenderSlot *render_slot_create() { return new RenderSlot(); }
 void render_slot_destroy(lyap_ctx *ctx, RenderSlot *slot)
 {
     if (!slot) return;
+    if (ctx->last_counters == slot->res.counters) ctx->last_counters = nullptr;
     release_resources(ctx, slot->res);
     delete slot;
 }
@@ -2092,7 +2095,10 @@ int render_slot_launch(lyap_ctx *ctx, RenderSlot *slot, const LyapView *in,
     uint32_t total_tiles = 0;
     int rc = prepare_views(ctx, slot->res, views.data(), n, plan, W, false, &total_tiles);
     if (rc) return rc;
-    return enqueue_views(ctx, slot->res, views.data(), n, total_tiles, plan, W, mode.precision);
+    rc = enqueue_views(ctx, slot->res, views.data(), n, total_tiles, plan, W, mode.precision);
+    if (rc) return rc;
+    ctx->last_counters = slot->res.counters;
+    return LYAP_OK;
 }
 
 } /* namespace lyap */
@@ -2328,6 +2334,10 @@ extern "C" void lyap_frame_destroy(lyap_frame *f)
     if (f->ctx) (void)hipStreamSynchronize(f->ctx->stream);
     if (f->exec) (void)hipGraphExecDestroy(f->exec);
     if (f->graph) (void)hipGraphDestroy(f->graph);
-    if (f->ctx) release_resources(f->ctx, f->res);
+    if (f->ctx) {
+        /* a later lyap_last_evals must not read the freed counters */
+        if (f->ctx->last_counters == f->res.counters) f->ctx->last_counters = nullptr;
+        release_resources(f->ctx, f->res);
+    }
     delete f;
 }

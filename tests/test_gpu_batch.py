@@ -148,26 +148,32 @@ def test_autotune_keeps_results(ctx):
         same(b.fetch(), e)
 
 
+@pytest.mark.parametrize("margin", [0.0, 0.01])
 @pytest.mark.parametrize("lanes,slots,tile,n", [(1, 1, 128, 3), (1, 1, 200, 3), (4, 1, 32, 3),
                                                 (8, 2, 8, 3), (16, 1, 4, 3), (2, 4, 64, 3),
                                                 (1, 1, 200, 5)])
-def test_launch_shapes_same_bytes(ctx, lanes, slots, tile, n):
+def test_launch_shapes_same_bytes(ctx, lanes, slots, tile, n, margin):
     """Lanes per ray, samples per lane and tile length change only the
     schedule: a tile longer than a wave is set up in chunks of 64 rays, a
     short one leaves groups to take rays from the next claim. Five views
-    include the 20-letter sequence (the 32-slot window)."""
+    include the 20-letter sequence (the 32-slot window). With an FP64
+    margin, rays refined in place (one lane per ray) and rays parked for the
+    refinement launch (two or more) re-evaluate the same bisection samples
+    in FP64, so the bytes and the evaluation count agree as well."""
     views, lights, n_lights = small_batch(n=n, seed=11)
     ld = device_lights(lights, n_lights)
-    mode = W.c2_mode(fp64_margin=0.0)
+    mode = W.c2_mode(fp64_margin=margin)
     bufs = alloc(views)
     api = as_api(views, bufs, ld, n_lights)
     ctx.render_batch(api, mode)
     ctx.sync()
+    ref_evals = ctx.last_evals()
     ref = [b.fetch() for b in bufs]
     t = lyap.Tuning()
     t.lanes_per_ray, t.slots, t.tile_rays = lanes, slots, tile
     ctx.render_batch(api, mode, t)
     ctx.sync()
+    assert ctx.last_evals() == ref_evals
     for b, e in zip(bufs, ref):
         same(b.fetch(), e)
 
@@ -188,14 +194,17 @@ def test_batch_errors(ctx):
     assert e.value.status == lyap.ERR_INVALID_ARGUMENT
 
 
-@pytest.mark.parametrize("prec", [lyap.PREC_PLAIN, lyap.PREC_PRODUCT, lyap.PREC_FP64])
-def test_direct_schedule_same_bytes(ctx, prec):
-    """The reference's launch shape (one lane per ray, no refill) and the
-    batched schedule compute every ray identically."""
+@pytest.mark.parametrize("prec,margin", [(lyap.PREC_PLAIN, 0.0), (lyap.PREC_PRODUCT, 0.0),
+                                         (lyap.PREC_FP64, 0.0), (lyap.PREC_PRODUCT, 0.01)])
+def test_direct_schedule_same_bytes(ctx, prec, margin):
+    """The reference's launch shape (one lane per ray, no refill, refinement
+    in place) and the batched schedule (rays parked for the refinement
+    launch) compute every ray identically, with and without FP64
+    re-evaluation of bisection samples near the threshold."""
     views, lights, n_lights = small_batch(6, seed=3)
     ld = device_lights(lights, n_lights)
-    batched = W.c2_mode(precision=prec, fp64_margin=0.0)
-    direct = W.c2_mode(precision=prec, fp64_margin=0.0)
+    batched = W.c2_mode(precision=prec, fp64_margin=margin)
+    direct = W.c2_mode(precision=prec, fp64_margin=margin)
     direct.schedule = lyap.SCHEDULE_DIRECT
     for v in views:
         a = RenderBuffers(v["w"], v["h"])
@@ -228,3 +237,28 @@ def test_bad_schedule(ctx):
     with pytest.raises(lyap.LyapError) as e:
         b.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, m)
     assert e.value.status == lyap.ERR_INVALID_ARGUMENT
+
+
+def test_last_evals_after_capture_and_destroy(ctx):
+    """Capturing a frame runs nothing, so the evaluation count still reads
+    the previous call's; a replay reports its own; after the frame is
+    destroyed the count no longer reads the frame's freed counters."""
+    views, lights, n_lights = small_batch(n=4, seed=5)
+    ld = device_lights(lights, n_lights)
+    mode = W.c2_mode(fp64_margin=0.0)
+    bufs = alloc(views)
+    api = as_api(views, bufs, ld, n_lights)
+    ctx.render_batch(api, mode)
+    ctx.sync()
+    e0 = ctx.last_evals()
+    assert e0 > 0
+    frame = ctx.frame_capture(api, mode)
+    assert ctx.last_evals() == e0
+    frame.replay()
+    ctx.sync()
+    assert ctx.last_evals() == e0
+    frame.close()
+    assert ctx.last_evals() in (0, e0)
+    ctx.render_batch(api, mode)
+    ctx.sync()
+    assert ctx.last_evals() == e0
