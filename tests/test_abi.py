@@ -104,3 +104,22 @@ def test_struct_layouts_match_c(tmp_path):
               "LyapEditor.num_lights": lyap.Editor.num_lights, "LyapEditor.pending": lyap.Editor.pending}
     for k, f in fields.items():
         assert f.offset == c[k][0], k
+
+
+def test_cpp_driver_builds_and_reports_no_device():
+    """The C++ driver (tests/driver) compiles against lyap.h and links
+    liblyap_amd.so; without a GPU it exits with LYAP_ERR_NO_DEVICE's code."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    d = os.path.join(ROOT, "tests", "driver")
+    subprocess.run(["make", "-C", d], check=True, capture_output=True)
+    r = subprocess.run([os.path.join(d, "lyap_driver"), "calculate", "8", "plain", os.devnull],
+                       capture_output=True, text=True)
+    assert r.returncode == -lyap.ERR_NO_DEVICE, r.stderr
+    assert "no device" in r.stderr
+    r = subprocess.run([os.path.join(d, "lyap_driver")], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr

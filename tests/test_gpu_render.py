@@ -4,10 +4,11 @@
    sample, so march decisions, sample counts, hit positions and depths agree
    exactly for both refinement methods (the reference walk with jitter and
    the near/far step, and the configured bisection);
- - the FP32 product form on the plain-render workload (512x512, AABAB,
-   100 + 300 iterations, 500 uniform steps, 8 bisections, h = 1e-3) is held to
-   the surface-depth bar: hit/miss agreement >= 99% and, where both hit,
-   depth within one march step on >= 99% of rays;
+ - the FP32 forms on the plain-render workload (512x512, AABAB, 100 + 300
+   iterations, 500 uniform steps, 8 bisections, h = 1e-3) are held to the
+   reference's own FP32 build on the same rays: hit/miss agreement and depth
+   within one march step of the FP64 oracle at least the oracle's
+   FP32-as-written form's, less 0.002;
  - the analytic one-letter scene puts the surface at x = 2 + e^-0.5 with
    normal +x;
  - the device's shading of its own hit records matches the oracle's shade()
@@ -108,39 +109,65 @@ def c2_oracle():
     return O.render(cam, prm, seq, lights, n_lights, w, h, mode=W.c2_mode(), lam=O.F64)
 
 
-@pytest.mark.parametrize("prec", [lyap.PREC_FP64, lyap.PREC_PRODUCT, lyap.PREC_PLAIN])
-def test_c2_depth_within_one_step(ctx, prec, c2_oracle):
-    """FP64: every sampled ray agrees with the oracle (hit, and depth within
-    one step). FP32 forms: a finite-window lambda near a chaotic transient
-    is a different realisation in FP32 than in FP64, and a ray samples ~26
-    points, so ~10% of rays stop a step or more apart (the reference's FP32
-    build behaves the same); the floor below is the measured level."""
-    w = h = 512
-    cam, prm, seq, lights, n_lights = W.render_config(w, h)
-    got = gpu_render(ctx, cam, prm, seq, lights, n_lights, w, h, W.c2_mode(precision=prec))
-    rows = (0, 1)
-    ref = c2_oracle
-    ys = np.arange(rows[0], h, rows[1])
-    g_d = got["depth"][ys]
-    r_d = ref["depth"][ys]
-    hit_g, hit_r = ~np.isnan(g_d), ~np.isnan(r_d)
+# Slack below the reference's own FP32 agreement that an FP32 form may sit
+# at: a different FP32 realisation of the same chaotic transients.
+FP32_SLACK = 0.002
+
+
+def depth_agreement(cam, prm, got, ref):
+    """(hit/miss agreement, fraction of rays both hit with depth within one
+    march step) of `got` against `ref`."""
+    hit_g, hit_r = ~np.isnan(got["depth"]), ~np.isnan(ref["depth"])
     agree = float(np.mean(hit_g == hit_r))
     both = hit_g & hit_r
     yy, xx = np.nonzero(both)
-    dt = step_of(cam, prm, w, h, ys[yy], xx)
-    within = float(np.mean(np.abs(g_d[both] - r_d[both]) <= dt * (1 + 1e-5)))
-    print(f"c2 prec {prec}: hit fraction {hit_r.mean():.3f} agree {agree:.4f} within-one-step {within:.4f}")
+    dt = step_of(cam, prm, 0, 0, yy, xx)
+    within = float(np.mean(np.abs(got["depth"][both] - ref["depth"][both]) <= dt * (1 + 1e-5)))
+    return agree, within
+
+
+@pytest.fixture(scope="module")
+def c2_reference_fp32(c2_oracle):
+    """The reference's FP32 build (the oracle's FP32-as-written form, whose
+    map and derivative promote to double through kernel.cu:135,143-144's
+    literals) against its FP64 build on the same frame: on this frame
+    hit/miss agreement 0.9986 and depth within one step 0.8721."""
+    w = h = 512
+    cam, prm, seq, lights, n_lights = W.render_config(w, h)
+    f32 = O.render(cam, prm, seq, lights, n_lights, w, h, mode=W.c2_mode(), lam=O.F32W)
+    return depth_agreement(cam, prm, f32, c2_oracle)
+
+
+@pytest.mark.parametrize("prec,margin", [(lyap.PREC_FP64, 0.0), (lyap.PREC_PRODUCT, 0.01),
+                                         (lyap.PREC_PRODUCT, 0.0), (lyap.PREC_PLAIN, 0.0)])
+def test_c2_depth_within_one_step(ctx, prec, margin, c2_oracle, c2_reference_fp32):
+    """The 512x512 plain render against the FP64 oracle, every ray. FP64:
+    every ray agrees (hit, depth, sample count). FP32 forms, including the
+    benchmarked one (product, no FP64 margin): a finite-window lambda near a
+    chaotic transient is a different realisation in FP32 than in FP64, and a
+    ray samples ~26 points, so ~13% of rays stop a step or more apart; the
+    floor is the reference's own FP32 build on the same rays (the oracle's
+    FP32-as-written form against FP64) less FP32_SLACK."""
+    w = h = 512
+    cam, prm, seq, lights, n_lights = W.render_config(w, h)
+    got = gpu_render(ctx, cam, prm, seq, lights, n_lights, w, h, W.c2_mode(precision=prec, fp64_margin=margin))
+    ref = c2_oracle
+    agree, within = depth_agreement(cam, prm, got, ref)
+    ref_agree, ref_within = c2_reference_fp32
+    hit_r = ~np.isnan(ref["depth"])
+    print(f"c2 prec {prec} margin {margin}: hit fraction {hit_r.mean():.3f} agree {agree:.4f} "
+          f"within-one-step {within:.4f} (reference FP32: {ref_agree:.4f} / {ref_within:.4f})")
     assert hit_r.mean() > 0.1
     if prec == lyap.PREC_FP64:
         assert agree == 1.0 and within == 1.0
-        assert np.array_equal(got["evals"][ys], ref["evals"][ys])
+        assert np.array_equal(got["evals"], ref["evals"])
     else:
-        assert agree >= 0.99
-        assert within >= 0.85
+        assert agree >= ref_agree - FP32_SLACK
+        assert within >= ref_within - FP32_SLACK
     # misses are transparent black with a NaN record
-    miss = ~hit_g
-    assert np.all(got["rgba"][ys][miss] == 0)
-    assert np.all(np.isnan(got["points"][ys][miss][:, 6]))
+    miss = np.isnan(got["depth"])
+    assert np.all(got["rgba"][miss] == 0)
+    assert np.all(np.isnan(got["points"][miss][:, 6]))
     assert got["total_evals"] == int(got["evals"].sum())
 
 

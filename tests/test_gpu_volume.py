@@ -2,13 +2,19 @@
 
 Parity bar (floating point, lambda compared rather than chaotic iterates):
  - LYAP_PREC_FP64 reproduces the oracle's FP64 orbit, so every voxel agrees
-   to 1e-9 and NaN positions coincide;
- - FP32 forms: on stable voxels (FP64 lambda < -1e-3 with the two halves of
-   the window agreeing to 1e-4, i.e. the orbit has settled) |dlambda| <= 1e-3
-   on >= 99% (plain) / 99.9% (product with FP64 re-evaluation); lambda sign
-   agreement >= 99% (plain) / 99.9% (re-evaluated).
-FP32 rounding can prolong a chaotic transient or select a coexisting
-attractor; the reference's own FP32 build shows the same effect.
+   to 1e-6 and NaN positions coincide;
+ - FP32 forms without re-evaluation: on stable voxels (FP64 lambda < -1e-3
+   with the two halves of the window agreeing to 1e-4, i.e. the orbit has
+   settled) the fraction within |dlambda| <= 1e-3, and the lambda sign
+   agreement over all voxels, are at least those of the reference's own
+   FP32 build on the same voxels (the oracle's FP32-as-written form against
+   its FP64 form) less FP32_SLACK. FP32 rounding can prolong a chaotic
+   transient or select a coexisting attractor, and the reference's FP32
+   build does so as often: on C1 it reaches 0.9963 within 1e-3 and 0.9939
+   sign agreement, on a C3 sample 0.9944 and 0.9914, short of the 0.999
+   bars;
+ - the product form with FP64 re-evaluation of uncertain voxels reaches the
+   0.999 bars on both.
 """
 import math
 
@@ -64,6 +70,23 @@ def c1_ref():
     return c1_reference()
 
 
+FP32_SLACK = 0.002
+
+
+@pytest.fixture(scope="module")
+def c1_reference_fp32(c1_ref):
+    """stats of the reference's FP32 build on C1 (within 0.9963, sign 0.9939)."""
+    ref, h1, h2 = c1_ref
+    prm, seq, lo, hi, n = W.volume_config("c1")
+    f32 = O.volume(prm, seq, lo, hi, n, mode=O.F32W)
+    return stats(f32, ref, h1, h2)
+
+
+def assert_at_reference_level(s, r):
+    assert s["within"] >= r["within"] - FP32_SLACK, (s, r)
+    assert s["sign"] >= r["sign"] - FP32_SLACK, (s, r)
+
+
 def test_c1_fp64_matches_oracle_exactly(ctx, c1_ref):
     prm, seq, lo, hi, n = W.volume_config("c1")
     ref = O.volume(prm, seq, lo, hi, n, mode=O.F64)   # rounded to float like the GPU output
@@ -73,22 +96,22 @@ def test_c1_fp64_matches_oracle_exactly(ctx, c1_ref):
     assert np.max(np.abs(got[m].astype(np.float64) - ref[m])) <= 1e-6
 
 
-def test_c1_plain(ctx, c1_ref):
+def test_c1_plain(ctx, c1_ref, c1_reference_fp32):
     ref, h1, h2 = c1_ref
     prm, seq, lo, hi, n = W.volume_config("c1")
     got = gpu_volume(ctx, prm, seq, lo, hi, n)
     s = stats(got, ref, h1, h2)
-    print("c1 plain", s)
-    assert s["within"] >= 0.99 and s["sign"] >= 0.99
+    print("c1 plain", s, "reference FP32", c1_reference_fp32)
+    assert_at_reference_level(s, c1_reference_fp32)
 
 
-def test_c1_product(ctx, c1_ref):
+def test_c1_product(ctx, c1_ref, c1_reference_fp32):
     ref, h1, h2 = c1_ref
     prm, seq, lo, hi, n = W.volume_config("c1")
     got = gpu_volume(ctx, prm, seq, lo, hi, n, lyap.volume_mode(lyap.PREC_PRODUCT))
     s = stats(got, ref, h1, h2)
-    print("c1 product", s)
-    assert s["within"] >= 0.99 and s["sign"] >= 0.99
+    print("c1 product", s, "reference FP32", c1_reference_fp32)
+    assert_at_reference_level(s, c1_reference_fp32)
 
 
 def test_c1_product_reevaluated(ctx, c1_ref):
@@ -99,6 +122,35 @@ def test_c1_product_reevaluated(ctx, c1_ref):
     s = stats(got, ref, h1, h2)
     print("c1 product+fp64", s, "reevaluated", nre / got.size)
     assert s["within"] >= 0.999 and s["sign"] >= 0.999
+
+
+def c3_sample():
+    prm, seq, lo, hi, n = W.volume_config("c3")
+    rng = np.random.default_rng(3)
+    idx = rng.integers(0, n, size=(50000, 3))
+    c = O.grid_coords(lo[0], hi[0], n)
+    xyz = np.stack([c[idx[:, 0]], c[idx[:, 1]], c[idx[:, 2]]], -1)
+    ref = O.lyap_points(xyz, prm.d, prm.settle, prm.accum, seq)
+    h1, h2 = O.lyap_halves(xyz, prm.d, prm.settle, prm.accum, prm.accum // 2, seq)
+    f32 = O.lyap_points(xyz, prm.d, prm.settle, prm.accum, seq, mode=O.F32W).astype(np.float32)
+    return idx, ref, h1, h2, f32
+
+
+@pytest.mark.parametrize("prec", [lyap.PREC_PLAIN, lyap.PREC_PRODUCT])
+def test_c3_fp32_at_reference_level(ctx, prec):
+    """The 256^3 precision-form volume in FP32 without re-evaluation, on
+    50k seeded voxels against the FP64 oracle: at the level of the
+    reference's FP32 build on the same voxels (0.9944 within 1e-3, 0.9914
+    sign agreement) less FP32_SLACK."""
+    prm, seq, lo, hi, n = W.volume_config("c3")
+    idx, ref, h1, h2, f32 = c3_sample()
+    got = gpu_volume(ctx, prm, seq, lo, hi, n, lyap.volume_mode(prec))
+    g = got[idx[:, 2], idx[:, 1], idx[:, 0]]
+    s = stats(g, ref, h1, h2)
+    r = stats(f32, ref, h1, h2)
+    print("c3", prec, s, "reference FP32", r)
+    assert r["sign"] < 0.999          # the reference's FP32 misses the sign bar itself
+    assert_at_reference_level(s, r)
 
 
 def test_c3_sampled(ctx):

@@ -361,3 +361,41 @@ def test_jitter_and_near_step_on_a_constant_lambda_ray(x0, lam_mode):
         c2 += 1
     assert out["evals"][8, 8] == c2
     assert count < c2 if lam > 0 else count > c2
+
+
+def _agreement(got, ref, h1, h2):
+    stable = (ref < -1e-3) & (np.abs(h1 - h2) < 1e-4)
+    d = np.abs(got[stable].astype(np.float64) - ref[stable])
+    within = float(np.mean(np.nan_to_num(d, nan=9.0) <= 1e-3))
+    sign = float(np.mean((np.sign(got) == np.sign(ref)) | (np.isnan(got) & np.isnan(ref))))
+    return within, sign
+
+
+def test_reference_fp32_is_short_of_the_bars():
+    """The reference's FP32 build (kernel.cu:135,143-144 as written: float
+    storage, double temporaries, logf per step) against its FP64 build, on
+    the 64^3 volume (C1) and on 50k seeded voxels of the 256^3 precision
+    volume (C3): within 1e-3 on stable voxels 0.9963 / 0.9944 and sign
+    agreement 0.9939 / 0.9914. Both sign figures are below the 0.999 bar,
+    which FP32 arithmetic of the reference's own method therefore does not
+    reach; the GPU tests hold the FP32 forms to these levels."""
+    from tests import workloads as W
+    prm, seq, lo, hi, n = W.volume_config("c1")
+    f64 = O.volume(prm, seq, lo, hi, n, mode=O.F64).astype(np.float64)
+    f32 = O.volume(prm, seq, lo, hi, n, mode=O.F32W)
+    c = O.grid_coords(lo[0], hi[0], n)
+    zz, yy, xx = np.meshgrid(c, c, c, indexing="ij")
+    xyz = np.stack([xx, yy, zz], -1).reshape(-1, 3)
+    h1, h2 = O.lyap_halves(xyz, prm.d, prm.settle, prm.accum, prm.accum // 2, seq)
+    w1, s1 = _agreement(f32, f64, h1.reshape(f64.shape), h2.reshape(f64.shape))
+    prm, seq, lo, hi, n = W.volume_config("c3")
+    idx = np.random.default_rng(3).integers(0, n, size=(50000, 3))
+    c = O.grid_coords(lo[0], hi[0], n)
+    xyz = np.stack([c[idx[:, 0]], c[idx[:, 1]], c[idx[:, 2]]], -1)
+    ref = O.lyap_points(xyz, prm.d, prm.settle, prm.accum, seq)
+    g = O.lyap_points(xyz, prm.d, prm.settle, prm.accum, seq, mode=O.F32W).astype(np.float32)
+    h1, h2 = O.lyap_halves(xyz, prm.d, prm.settle, prm.accum, prm.accum // 2, seq)
+    w3, s3 = _agreement(g, ref, h1, h2)
+    assert abs(w1 - 0.9963) < 5e-4 and abs(s1 - 0.9939) < 5e-4, (w1, s1)
+    assert abs(w3 - 0.9944) < 5e-4 and abs(s3 - 0.9914) < 5e-4, (w3, s3)
+    assert s1 < 0.999 and s3 < 0.999
