@@ -161,6 +161,24 @@ int lyap_calc_volume(lyap_ctx *ctx, float *d_exps, const LyapParams *prm,
                      unsigned nx, unsigned ny, unsigned nz,
                      const LyapVolumeMode *mode);
 
+/* Allocation-free variant of lyap_calc_volume. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, kernels and results as lyap_calc_volume, but
+ * the FP64 re-evaluation list and its counter live in a fixed per-device
+ * array of 16 MiB that the library holds until the process ends, and the
+ * grid is processed in z slabs that the list holds entirely. Limit: one z
+ * slice (nx*ny voxels) of at most LYAP_NOALLOC_MAX_SLICE, else
+ * LYAP_ERR_UNSUPPORTED. Nothing on this path calls hipMalloc or hipFree.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (two streams, or two host threads) race and must not be made.
+ * The call does not update lyap_last_reevaluated. */
+#define LYAP_NOALLOC_MAX_SLICE 4194304u
+int lyap_calc_volume_noalloc(lyap_ctx *ctx, float *d_exps, const LyapParams *prm,
+                             const int32_t *seq, const float lo[3], const float hi[3],
+                             unsigned nx, unsigned ny, unsigned nz,
+                             const LyapVolumeMode *mode);
+
 /* lambda at n arbitrary points (x, y, z float triples in device memory):
  * lyap4d (kernel.hpp:13, kernel.cu:108-154) as a batch entry. */
 int lyap_calc_points(lyap_ctx *ctx, float *d_out, const float *d_xyz, size_t n,

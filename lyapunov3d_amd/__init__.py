@@ -237,6 +237,7 @@ _SIGS = {
     "lyap_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "lyap_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "lyap_calc_volume": (C.c_int, [_P, _P, C.POINTER(LyapParams), C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint, C.c_uint, C.c_uint, C.POINTER(VolumeMode)]),
+    "lyap_calc_volume_noalloc": (C.c_int, [_P, _P, C.POINTER(LyapParams), C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint, C.c_uint, C.c_uint, C.POINTER(VolumeMode)]),
     "lyap_calc_points": (C.c_int, [_P, _P, _P, C.c_size_t, C.POINTER(LyapParams), C.POINTER(C.c_int32), C.POINTER(VolumeMode)]),
     "lyap_last_reevaluated": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "lyap_calc_points_halves": (C.c_int, [_P, _P, _P, _P, _P, C.c_size_t, C.POINTER(LyapParams),
@@ -469,6 +470,17 @@ class Context:
         rc = self.lib.lyap_calc_volume(self.h, out_ptr, C.byref(prm), seq_array(seq), lo3, hi3,
                                        nx, ny, nz, C.byref(mode) if mode is not None else None)
         _check(rc, self.h, "lyap_calc_volume")
+
+    def calc_volume_noalloc(self, out_ptr: int, prm: LyapParams, seq, lo, hi, n,
+                            mode: Optional[VolumeMode] = None):
+        """lyap_calc_volume_noalloc: calc_volume with no device allocation
+        (a fixed per-device scratch array; one z slice at most 4,194,304 voxels)."""
+        nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+        lo3 = (C.c_float * 3)(*lo)
+        hi3 = (C.c_float * 3)(*hi)
+        rc = self.lib.lyap_calc_volume_noalloc(self.h, out_ptr, C.byref(prm), seq_array(seq), lo3, hi3,
+                                               nx, ny, nz, C.byref(mode) if mode is not None else None)
+        _check(rc, self.h, "lyap_calc_volume_noalloc")
 
     def calc_points(self, out_ptr: int, xyz_ptr: int, n: int, prm: LyapParams, seq,
                     mode: Optional[VolumeMode] = None):

@@ -139,6 +139,9 @@ int lyap_create(lyap_ctx **out, int device, size_t hbm_budget_bytes)
         return LYAP_ERR_DEVICE;
     }
     ctx->counters = (uint32_t *)p;
+    /* the allocation-free entry's scratch address on this device; a failure
+     * is reported by that entry alone */
+    (void)noalloc_scratch_setup(device);
     if (launch_zero_u32(ctx, ctx->counters, 64) != LYAP_OK ||
         hipStreamSynchronize(ctx->stream) != hipSuccess) {
         lyap_destroy(ctx);
@@ -224,6 +227,25 @@ int lyap_calc_volume(lyap_ctx *ctx, float *d_exps, const LyapParams *prm,
     int rc = check_params(ctx, prm, seq, &e, mode.renorm);
     if (rc) return rc;
     return launch_volume(ctx, d_exps, e, lo, hi, nx, ny, nz, nullptr, 0, mode);
+}
+
+int lyap_calc_volume_noalloc(lyap_ctx *ctx, float *d_exps, const LyapParams *prm,
+                             const int32_t *seq, const float lo[3], const float hi[3],
+                             unsigned nx, unsigned ny, unsigned nz,
+                             const LyapVolumeMode *mode_in)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    LyapVolumeMode mode{};
+    if (mode_in) mode = *mode_in;
+    if (mode.precision < LYAP_PREC_PLAIN || mode.precision > LYAP_PREC_FP64)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown precision %d", mode.precision);
+    if (!lo || !hi) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null box");
+    if ((uint64_t)nx * ny * nz == 0) return LYAP_OK;
+    if (!d_exps) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null output");
+    EvalDesc e;
+    int rc = check_params(ctx, prm, seq, &e, mode.renorm);
+    if (rc) return rc;
+    return launch_volume_noalloc(ctx, d_exps, e, lo, hi, nx, ny, nz, mode);
 }
 
 int lyap_calc_points(lyap_ctx *ctx, float *d_out, const float *d_xyz, size_t n,

@@ -70,6 +70,15 @@ int launch_volume_slab(lyap_ctx *ctx, float *out, const EvalDesc &e,
                        unsigned ny, unsigned nz, unsigned z_begin, unsigned z_end,
                        const float *xyz, size_t npoints, const LyapVolumeMode &mode);
 
+/* The allocation-free volume path: entries of its scratch list (16 MiB). */
+constexpr uint32_t kNoAllocListCap = 1u << 22;
+/* Looks up the scratch array's address on `device` (current) once per
+ * device; lyap_create calls it. */
+int noalloc_scratch_setup(int device);
+int launch_volume_noalloc(lyap_ctx *ctx, float *out, const EvalDesc &e, const float lo[3],
+                          const float hi[3], unsigned nx, unsigned ny, unsigned nz,
+                          const LyapVolumeMode &mode);
+
 int launch_points_halves(lyap_ctx *ctx, float *lam, float *h1, float *h2, const EvalDesc &e,
                          const float *xyz, size_t n);
 

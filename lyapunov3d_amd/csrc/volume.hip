@@ -12,6 +12,8 @@
  * to a compact list (one atomic per wave) and a second kernel walks that list
  * with the FP64 form, so the FP64 work runs at full lane occupancy.
  */
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace lyap {
@@ -224,11 +226,14 @@ int launch_points_halves(lyap_ctx *ctx, float *lam, float *h1, float *h2, const 
     }
 }
 
-/* z slices [z_begin, z_end) of the grid; out points at slice z_begin. */
-int launch_volume_slab(lyap_ctx *ctx, float *out, const EvalDesc &e,
-                       const float lo[3], const float hi[3], unsigned nx,
-                       unsigned ny, unsigned nz, unsigned z_begin, unsigned z_end,
-                       const float *xyz, size_t npoints, const LyapVolumeMode &mode)
+/* z slices [z_begin, z_end) of the grid; out points at slice z_begin. The
+ * FP64 re-evaluation list (list_cap entries) and its counter are given:
+ * nothing here allocates. The counter is cleared by a kernel first. */
+static int launch_volume_slab_in(lyap_ctx *ctx, float *out, const EvalDesc &e,
+                                 const float lo[3], const float hi[3], unsigned nx,
+                                 unsigned ny, unsigned nz, unsigned z_begin, unsigned z_end,
+                                 const float *xyz, size_t npoints, const LyapVolumeMode &mode,
+                                 uint32_t *list, uint32_t list_cap, uint32_t *count)
 {
     VolumeArgs a{};
     a.out = out;
@@ -256,14 +261,12 @@ int launch_volume_slab(lyap_ctx *ctx, float *out, const EvalDesc &e,
     const bool reeval = mode.precision == LYAP_PREC_PRODUCT &&
                         (mode.fp64_margin > 0.0f || mode.converge_tau > 0.0f);
     const bool halves = reeval && mode.converge_tau > 0.0f;
-    a.count = ctx->counters;
-    int rc = launch_zero_u32(ctx, ctx->counters, 1);
+    a.count = count;
+    int rc = launch_zero_u32(ctx, count, 1);
     if (rc) return rc;
     if (reeval) {
-        rc = ensure_reeval_list(ctx, a.n);
-        if (rc) return rc;
-        a.list = ctx->reeval_list;
-        a.list_cap = (uint32_t)ctx->reeval_cap;
+        a.list = list;
+        a.list_cap = list_cap;
     }
     switch (e.L) {
         LYAP_L_CASES(VolumeLaunch, ctx, a, mode.precision, halves, reeval)
@@ -271,6 +274,87 @@ int launch_volume_slab(lyap_ctx *ctx, float *out, const EvalDesc &e,
         return fail(ctx, LYAP_ERR_BAD_SEQUENCE, "sequence period %d outside 1..%d",
                     e.L, LYAP_MAX_SEQ);
     }
+}
+
+int launch_volume_slab(lyap_ctx *ctx, float *out, const EvalDesc &e,
+                       const float lo[3], const float hi[3], unsigned nx,
+                       unsigned ny, unsigned nz, unsigned z_begin, unsigned z_end,
+                       const float *xyz, size_t npoints, const LyapVolumeMode &mode)
+{
+    const uint64_t n = xyz ? (uint64_t)npoints : (uint64_t)nx * ny * (z_end - z_begin);
+    const bool reeval = mode.precision == LYAP_PREC_PRODUCT &&
+                        (mode.fp64_margin > 0.0f || mode.converge_tau > 0.0f);
+    if (reeval && n > 0 && n <= 0xFFFFFFFFull) {
+        int rc = ensure_reeval_list(ctx, n);
+        if (rc) return rc;
+    }
+    return launch_volume_slab_in(ctx, out, e, lo, hi, nx, ny, nz, z_begin, z_end, xyz, npoints,
+                                 mode, ctx->reeval_list, (uint32_t)ctx->reeval_cap, ctx->counters);
+}
+
+/* ---- the allocation-free volume entry ------------------------------------- */
+
+/* Scratch of lyap_calc_volume_noalloc: word 0 is the FP64 re-evaluation
+ * counter (alone on its 256-byte line), the list starts at word
+ * kNoAllocListOffset. One copy per device, held until the process ends. In
+ * host code this name is only a stand-in: kernels receive the device
+ * address that hipGetSymbolAddress returned for the device
+ * (noalloc_scratch_setup). */
+constexpr uint32_t kNoAllocListOffset = 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocListOffset + kNoAllocListCap];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch[kMaxScratchDevices];       /* device address, per device */
+int g_scratch_status[kMaxScratchDevices];      /* 0: not looked up, 1: ok, < 0: failed */
+}
+
+int noalloc_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return LYAP_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_status[device] != 0) return g_scratch_status[device] > 0 ? LYAP_OK : LYAP_ERR_DEVICE;
+    /* the caller has made `device` current */
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        g_scratch_status[device] = -1;
+        return LYAP_ERR_DEVICE;
+    }
+    g_scratch[device] = static_cast<uint32_t *>(p);
+    g_scratch_status[device] = 1;
+    return LYAP_OK;
+}
+
+int launch_volume_noalloc(lyap_ctx *ctx, float *out, const EvalDesc &e, const float lo[3],
+                          const float hi[3], unsigned nx, unsigned ny, unsigned nz,
+                          const LyapVolumeMode &mode)
+{
+    uint32_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxScratchDevices && g_scratch_status[ctx->device] > 0)
+            base = g_scratch[ctx->device];
+    }
+    if (!base)
+        return fail(ctx, LYAP_ERR_DEVICE, "the scratch array's device address is unknown "
+                    "(hipGetSymbolAddress failed when the context was created)");
+    const uint64_t plane = (uint64_t)nx * ny;
+    if (plane > kNoAllocListCap)
+        return fail(ctx, LYAP_ERR_UNSUPPORTED,
+                    "a z slice of %llu voxels exceeds the %u-entry scratch list",
+                    (unsigned long long)plane, kNoAllocListCap);
+    /* slabs of whole z slices that the list can hold entirely, so every voxel
+     * the rule selects is re-evaluated, as in lyap_calc_volume */
+    const uint32_t per = (uint32_t)(kNoAllocListCap / plane);
+    for (uint32_t z = 0; z < nz; z += per) {
+        const uint32_t z1 = nz - z < per ? nz : z + per;
+        int rc = launch_volume_slab_in(ctx, out + plane * z, e, lo, hi, nx, ny, nz, z, z1, nullptr, 0,
+                                       mode, base + kNoAllocListOffset, kNoAllocListCap, base);
+        if (rc) return rc;
+    }
+    return LYAP_OK;
 }
 
 } /* namespace lyap */
