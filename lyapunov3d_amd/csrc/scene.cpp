@@ -4,13 +4,16 @@
  * Restates params_init (params.cu:21-114), scene_convert_sequence
  * (scene.cu:69-108), scene_cam_recalculate (scene.cu:31-63),
  * scene_lights_recalculate (scene.cu:20-29) and the quaternion helpers they
- * use (quat.hpp:147-165, 194-238, 279-284). Differences from the reference:
+ * use (quat.hpp:147-165, 194-238, 279-284), in float with the reference's
+ * operation order, so the camera vectors (V, S0, SDX, SDY) and the cone
+ * cosines are bit for bit the reference's for the same inputs (this file is
+ * built with -ffp-contract=off: no multiply-add is fused). Differences from
+ * the reference:
  *  - errors are returned (a bad sequence letter exits the reference process);
- *  - quaternion normalisation divides by |q| (the reference multiplies,
- *    quat.hpp:159-163, which is only harmless for already-unit q);
- *  - quaternion rotation is evaluated in double and rounded once, so results
- *    can differ from the reference's float expression in the last bit;
  *  - a zero renderDenominator is treated as 1 (the reference divides by it).
+ * QUAT::normalize is restated with its quirk: outside a 1e-6 band around
+ * |q|^2 = 1 it multiplies by |q| (quat.hpp:159-163) instead of dividing;
+ * callers keep it a no-op by passing unit quaternions.
  */
 #include <cmath>
 #include <cstring>
@@ -19,30 +22,50 @@
 
 namespace {
 
-struct D3 { double x, y, z; };
+constexpr double kQuatEpsilon = 1e-6;        /* quat.hpp:12 */
+constexpr double kVecEpsilon = 1e-12;        /* vec3.hpp:10 */
+constexpr double kQuatPi = 3.14159265358979323846264338327950288;
 
-inline D3 to_d(const LyapVec &v) { return {v.x, v.y, v.z}; }
-inline LyapVec to_f(const D3 &v) { return {(float)v.x, (float)v.y, (float)v.z}; }
-
-/* Rotation of v by q = (x, y, z, w): the rotation matrix of q scaled by |q|^2,
- * the value QUAT::transform computes (quat.hpp:279-284). */
-D3 rotate(const LyapQuat &q, const D3 &v)
+/* QUAT::transform (quat.hpp:279-284): each component is a sum of eight
+ * float products, accumulated in the order the reference writes them
+ * (left to right), each product formed left to right too. */
+LyapVec qtransform(const LyapQuat &q, const LyapVec &v)
 {
-    const double x = q.x, y = q.y, z = q.z, w = q.w;
-    const double xx = x * x, yy = y * y, zz = z * z, ww = w * w;
-    return {
-        (ww + xx - yy - zz) * v.x + 2.0 * (x * y - w * z) * v.y + 2.0 * (x * z + w * y) * v.z,
-        2.0 * (x * y + w * z) * v.x + (ww - xx + yy - zz) * v.y + 2.0 * (y * z - w * x) * v.z,
-        2.0 * (x * z - w * y) * v.x + 2.0 * (y * z + w * x) * v.y + (ww - xx - yy + zz) * v.z,
-    };
+    const float x = q.x, y = q.y, z = q.z, w = q.w, two = 2.0f;
+    float ox = w * w * v.x;
+    ox += two * y * w * v.z;
+    ox -= two * z * w * v.y;
+    ox += x * x * v.x;
+    ox += two * y * x * v.y;
+    ox += two * z * x * v.z;
+    ox -= z * z * v.x;
+    ox -= y * y * v.x;
+    float oy = two * x * y * v.x;
+    oy += y * y * v.y;
+    oy += two * z * y * v.z;
+    oy += two * w * z * v.x;
+    oy -= z * z * v.y;
+    oy += w * w * v.y;
+    oy -= two * x * w * v.z;
+    oy -= x * x * v.y;
+    float oz = two * x * z * v.x;
+    oz += two * y * z * v.y;
+    oz += z * z * v.z;
+    oz -= two * w * y * v.x;
+    oz -= y * y * v.z;
+    oz += two * w * x * v.y;
+    oz -= x * x * v.z;
+    oz += w * w * v.z;
+    return {ox, oy, oz};
 }
 
-/* Vec::normalize on a float vector (vec3.hpp:141-154). */
+/* Vec::normalized (vec3.hpp:128-139): float |v|^2, zero below 1e-12, left
+ * alone within 1e-12 of 1 (only exactly 1 in float). */
 LyapVec vnormalized(LyapVec v)
 {
     const float m2 = v.x * v.x + v.y * v.y + v.z * v.z;
-    if ((double)m2 < 1e-12) return {0.f, 0.f, 0.f};
-    if (m2 == 1.0f) return v;
+    if ((double)m2 < kVecEpsilon) return {0.f, 0.f, 0.f};
+    if (m2 == 1.0f || ((double)m2 > 1.0 - kVecEpsilon && (double)m2 < 1.0 + kVecEpsilon)) return v;
     const float m = std::sqrt(m2);
     return {v.x / m, v.y / m, v.z / m};
 }
@@ -52,22 +75,28 @@ float vdot(const LyapVec &a, const LyapVec &b)
     return a.x * b.x + a.y * b.y + a.z * b.z;
 }
 
-/* QUAT::normalize (quat.hpp:147-165) with the division the code intends. */
+/* Vec operator* (vec3.hpp:112-115): the cross product. */
+LyapVec vcross(const LyapVec &a, const LyapVec &b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+/* QUAT::normalize (quat.hpp:147-165): float |q|^2; the identity below 1e-6;
+ * unchanged within 1e-6 of 1; otherwise each component times |q|. */
 void qnormalize(LyapQuat *q)
 {
-    const double m2 = (double)q->x * q->x + (double)q->y * q->y +
-                      (double)q->z * q->z + (double)q->w * q->w;
-    if (m2 < 1e-6) {
+    float m2 = q->x * q->x + q->y * q->y + q->z * q->z + q->w * q->w;
+    if ((double)m2 < kQuatEpsilon) {
         q->x = q->y = q->z = 0.0f;
         q->w = 1.0f;
         return;
     }
-    if (std::fabs(m2 - 1.0) < 1e-6) return;
-    const double m = std::sqrt(m2);
-    q->x = (float)(q->x / m);
-    q->y = (float)(q->y / m);
-    q->z = (float)(q->z / m);
-    q->w = (float)(q->w / m);
+    if (m2 == 1.0f || ((double)m2 > 1.0 - kQuatEpsilon && (double)m2 < 1.0 + kQuatEpsilon)) return;
+    m2 = std::sqrt(m2);
+    q->x *= m2;
+    q->y *= m2;
+    q->z *= m2;
+    q->w *= m2;
 }
 
 LyapColor color(float r, float g, float b, float a)
@@ -83,44 +112,45 @@ extern "C" {
 
 void lyap_quat_transform(const LyapQuat *q, const LyapVec *v, LyapVec *out)
 {
-    *out = to_f(rotate(*q, to_d(*v)));
+    *out = qtransform(*q, *v);
 }
 
+/* QUAT(axis, ang, inDegrees) (quat.hpp:227-238): the half angle formed in
+ * double from the double pi and stored as float, float sin/cos. */
 void lyap_quat_axis_angle(const LyapVec *axis, float angle, int in_degrees,
                           LyapQuat *out)
 {
-    const double half = in_degrees ? (double)angle * M_PI / 360.0
-                                   : (double)angle * 0.5;
-    const double s = std::sin(half);
-    out->x = (float)(axis->x * s);
-    out->y = (float)(axis->y * s);
-    out->z = (float)(axis->z * s);
-    out->w = (float)std::cos(half);
+    const float half = in_degrees ? (float)((double)angle * kQuatPi / (double)360.0f) : angle * 0.5f;
+    const float s = std::sin(half);
+    out->x = axis->x * s;
+    out->y = axis->y * s;
+    out->z = axis->z * s;
+    out->w = std::cos(half);
     qnormalize(out);
 }
 
+/* QUAT(p, q, scale) (quat.hpp:194-220), in float; the reference returns the
+ * identity when the vectors are (nearly) perpendicular, |cos| <= 1e-6
+ * (quat.hpp:204-208): kept as written. */
 void lyap_quat_between(const LyapVec *p, const LyapVec *q, float scale,
                        LyapQuat *out)
 {
-    double c = (double)p->x * q->x + (double)p->y * q->y + (double)p->z * q->z;
-    if (c < -1.0) c = -1.0;
-    else if (c > 1.0) c = 1.0;
-    /* The reference returns the identity when the vectors are (nearly)
-     * perpendicular, |cos| <= 1e-6 (quat.hpp:204-208); kept as written. */
-    if (c >= -1e-6 && c <= 1e-6) {
+    float c = vdot(*p, *q);
+    if ((double)c < -1.0) c = -1.0f;
+    else if ((double)c > 1.0) c = 1.0f;
+    if (c == 0.0f || ((double)c >= -kQuatEpsilon && (double)c <= kQuatEpsilon)) {
         out->x = out->y = out->z = 0.0f;
         out->w = 1.0f;
         return;
     }
-    const double ang = std::acos(c);
-    const D3 axis = {(double)p->y * q->z - (double)p->z * q->y,
-                     (double)p->z * q->x - (double)p->x * q->z,
-                     (double)p->x * q->y - (double)p->y * q->x};
-    const double k = std::sin(ang * 0.5 * scale) / std::sin(ang);
-    out->x = (float)(axis.x * k);
-    out->y = (float)(axis.y * k);
-    out->z = (float)(axis.z * k);
-    out->w = (float)std::cos(ang * 0.5 * scale);
+    const float ang = std::acos(c);
+    const LyapVec axis = vcross(*p, *q);
+    const float half = (float)((double)ang * 0.5 * (double)scale);
+    const float k = std::sin(half) / std::sin(ang);
+    out->x = axis.x * k;
+    out->y = axis.y * k;
+    out->z = axis.z * k;
+    out->w = std::cos(half);
 }
 
 void lyap_scene_cam_recalculate(LyapCam *cam, unsigned tw, unsigned th,
@@ -138,25 +168,26 @@ void lyap_scene_cam_recalculate(LyapCam *cam, unsigned tw, unsigned th,
         cam->textureHeight = th;
         cam->renderHeight = th / cam->renderDenominator;
     }
-    const double M = cam->M;
-    cam->V = vnormalized(to_f(rotate(cam->Q, {0.0, 0.0, 1.0})));
-    cam->S0 = to_f(rotate(cam->Q, {-M, -M, 1.0}));
-    cam->lightInnerCone = vdot(cam->V, vnormalized(to_f(rotate(cam->Q, {-M, -M, 1.5}))));
-    cam->lightOuterCone = vdot(cam->V, vnormalized(to_f(rotate(cam->Q, {-M, -M, 1.0}))));
-    const float sx = 2.0f * cam->M / (float)(cam->renderWidth ? cam->renderWidth : 1);
-    const float sy = 2.0f * cam->M / (float)(cam->renderHeight ? cam->renderHeight : 1);
-    cam->SDX = to_f(rotate(cam->Q, {sx, 0.0, 0.0}));
-    cam->SDY = to_f(rotate(cam->Q, {0.0, sy, 0.0}));
+    const float M = cam->M;
+    cam->V = vnormalized(qtransform(cam->Q, {0.0f, 0.0f, 1.0f}));
+    cam->S0 = qtransform(cam->Q, {-M, -M, 1.0f});
+    cam->lightInnerCone = vdot(cam->V, vnormalized(qtransform(cam->Q, {-M, -M, 1.5f})));
+    cam->lightOuterCone = vdot(cam->V, vnormalized(qtransform(cam->Q, {-M, -M, 1.0f})));
+    /* Vec(2*M / (Real)renderWidth, 0, 0) (scene.cu:57-62) */
+    const float sx = 2 * M / (float)(cam->renderWidth ? cam->renderWidth : 1);
+    const float sy = 2 * M / (float)(cam->renderHeight ? cam->renderHeight : 1);
+    cam->SDX = qtransform(cam->Q, {sx, 0.0f, 0.0f});
+    cam->SDY = qtransform(cam->Q, {0.0f, sy, 0.0f});
 }
 
 void lyap_scene_lights_recalculate(LyapLight *lights, size_t num_lights)
 {
     for (size_t i = 0; i < num_lights; ++i) {
         LyapLight *L = &lights[i];
-        const double M = L->M;
-        L->V = vnormalized(to_f(rotate(L->Q, {0.0, 0.0, 1.0})));
-        L->lightInnerCone = vdot(L->V, vnormalized(to_f(rotate(L->Q, {-M, -M, 1.5}))));
-        L->lightOuterCone = vdot(L->V, vnormalized(to_f(rotate(L->Q, {-M, -M, 1.0}))));
+        const float M = L->M;
+        L->V = vnormalized(qtransform(L->Q, {0.0f, 0.0f, 1.0f}));
+        L->lightInnerCone = vdot(L->V, vnormalized(qtransform(L->Q, {-M, -M, 1.5f})));
+        L->lightOuterCone = vdot(L->V, vnormalized(qtransform(L->Q, {-M, -M, 1.0f})));
     }
 }
 
@@ -220,12 +251,11 @@ void lyap_params_init(LyapParams *prm, LyapCam *cam, LyapLight *lights,
         const LyapVec dir = {4.0f, 4.0f, 4.0f};
         LyapVec side = vnormalized({-4.0f, 4.0f, 4.0f});
         const LyapVec dn = vnormalized(dir);
-        LyapVec up = {side.y * dn.z - side.z * dn.y, side.z * dn.x - side.x * dn.z,
-                      side.x * dn.y - side.y * dn.x};
-        up = vnormalized(up);
+        LyapVec up = vnormalized(vcross(side, dn));
+        /* nlerp(rot0, rot1, 1) is rot1 (quat.hpp:178-179) */
         LyapQuat rot;
         lyap_quat_axis_angle(&up, 20.0f, 1, &rot);
-        LyapVec nd = vnormalized(to_f(rotate(rot, to_d(dir))));
+        LyapVec nd = vnormalized(qtransform(rot, dir));
         nd = {-nd.x, -nd.y, -nd.z};
         const float c0 = (float)(4.0 - 0.9);
         cam->C = {c0 - nd.x, c0 - nd.y, c0 - nd.z};
@@ -259,16 +289,19 @@ const char *lyap_default_sequence(void) { return "BCABA"; }
 void lyap_cam_look_at(LyapCam *cam, const LyapVec *eye, const LyapVec *target,
                       float M, unsigned w, unsigned h)
 {
-    D3 f = {(double)target->x - eye->x, (double)target->y - eye->y,
-            (double)target->z - eye->z};
-    const double fl = std::sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
+    double fx = (double)target->x - eye->x, fy = (double)target->y - eye->y,
+           fz = (double)target->z - eye->z;
+    const double fl = std::sqrt(fx * fx + fy * fy + fz * fz);
     LyapQuat q;
     q.x = q.y = q.z = 0.0f;
     q.w = 1.0f;
     if (fl > 0.0) {
-        f = {f.x / fl, f.y / fl, f.z / fl};
-        /* shortest-arc rotation of +z onto f: q = (z x f, 1 + z.f), normalised */
-        double qx = -f.y, qy = f.x, qz = 0.0, qw = 1.0 + f.z;
+        fx /= fl;
+        fy /= fl;
+        fz /= fl;
+        /* shortest-arc rotation of +z onto f: q = (z x f, 1 + z.f), normalised
+         * in double, so QUAT::normalize leaves it alone */
+        double qx = -fy, qy = fx, qz = 0.0, qw = 1.0 + fz;
         if (qw < 1e-12) { qx = 1.0; qy = 0.0; qz = 0.0; qw = 0.0; }  /* f = -z */
         const double n = std::sqrt(qx * qx + qy * qy + qz * qz + qw * qw);
         q.x = (float)(qx / n);

@@ -18,7 +18,16 @@ the oracle's own loop, chosen so that a plausible slip fails one of them:
  - brute force: 60-digit mpmath iteration on stable points;
  - raymarch: the one-letter analytic scene (surface plane at x = 2 + e^-0.5)
    and the slab clip / pixel-corner ray in closed form;
- - shade: Blinn-Phong with camera, light and normal aligned, in closed form.
+ - shade: Blinn-Phong with camera, light and normal aligned, in closed form,
+   and each branch of kernel.cu:67-100: the ramp between the spot cones, the
+   ambient-only side outside the outer cone, the chaos term (including
+   c = 1, where 0.1125 / ln c is infinite), and the sum over lights;
+ - the reference's refinement walk (kernel.cu:402-441) on an axis-aligned
+   ray through the analytic plane with dyadic positions, its seven samples,
+   reversals and halvings worked out by hand;
+ - FP32 as written: a dyadic orbit where the double temporaries of
+   kernel.cu:135 round to a different float than float arithmetic would,
+   moving the iterate off 0.5 so the guard of kernel.cu:138 does not fire.
 """
 import math
 
@@ -297,6 +306,134 @@ def test_shade_closed_form():
     pt[6] = np.nan                           # miss -> transparent black
     col, rgba = O.shade(pt, cam, lights, 1)
     assert np.all(col == 0) and np.all(rgba == 0)
+
+
+def one_light_scene(outer=0.5, inner=0.9, axis=(0.0, 0.0, -1.0)):
+    """Point at the origin with normal +z, camera at (0,0,5), the default
+    light moved to (0,0,2) with spot axis `axis`: L = +z, H = +z, d^2 = 4."""
+    prm, cam, lights, n = lyap.params_init()
+    L = lights[0]
+    L.C = lyap.Vec(0.0, 0.0, 2.0)
+    L.V = lyap.Vec(*axis)
+    L.lightOuterCone, L.lightInnerCone = outer, inner
+    cam.C = lyap.Vec(0.0, 0.0, 5.0)
+    pt = np.zeros(9, np.float32)
+    pt[5] = 1.0
+    return cam, lights, L, pt
+
+
+def full_phong(L):
+    """(Cs ks + Cd kd) range / d^2 per channel, with L.N = N.H = 1, d^2 = 4."""
+    k = L.lightRange / 4.0
+    return np.array([(L.specularColor.x * L.specularPower + L.diffuseColor.x * L.diffusePower) * k,
+                     (L.specularColor.y * L.specularPower + L.diffuseColor.y * L.diffusePower) * k,
+                     (L.specularColor.z * L.specularPower + L.diffuseColor.z * L.diffusePower) * k,
+                     (L.specularColor.w * L.specularPower + L.diffuseColor.w * L.diffusePower) * k])
+
+
+def amb(L):
+    return np.array([L.ambient.x, L.ambient.y, L.ambient.z, L.ambient.w])
+
+
+def test_shade_cone_ramp():
+    """j = -L.V . L = 0.7 lies between the outer (0.5) and inner (0.9) cone
+    cosines: the Phong term is scaled by (0.7 - 0.5) / (0.9 - 0.5) = 1/2
+    before the ambient is added (kernel.cu:87-90)."""
+    cam, lights, L, pt = one_light_scene(axis=(0.0, math.sqrt(1 - 0.49), -0.7))
+    col, _ = O.shade(pt, cam, lights, 1)
+    np.testing.assert_allclose(col[0], full_phong(L) * 0.5 + amb(L), rtol=2e-6)
+
+
+def test_shade_outside_the_outer_cone_is_ambient():
+    """Spot axis pointing away (j = -1 <= outer): only the ambient colour
+    (kernel.cu:92-94), exactly."""
+    cam, lights, L, pt = one_light_scene(axis=(0.0, 0.0, 1.0))
+    L.ambient = lyap.Color(0.25, 0.125, 0.5, 0.75)
+    col, _ = O.shade(pt, cam, lights, 1)
+    assert col[0].tolist() == [0.25, 0.125, 0.5, 0.75]
+
+
+def test_shade_chaos_term_and_two_lights():
+    """c > 0 adds chaosColor * 0.1125 / ln c (kernel.cu:96-100): c = 4 adds
+    0.1125 / ln 4 of it; c = 1 makes the factor infinite, so a zero channel
+    gives NaN and a non-zero one infinity. Two lights add."""
+    cam, lights, L, pt = one_light_scene(axis=(0.0, 0.0, 1.0))
+    L.ambient = lyap.Color(0.25, 0.0, 0.0, 0.0)
+    L.chaosColor = lyap.Color(0.0, 0.5, 1.0, 0.0)
+    pt[7] = 4.0                                   # c
+    col, _ = O.shade(pt, cam, lights, 1)
+    k = 0.1125 / math.log(4.0)
+    np.testing.assert_allclose(col[0], [0.25, 0.5 * k, k, 0.0], rtol=2e-6)
+    pt[7] = 1.0
+    col, rgba = O.shade(pt, cam, lights, 1)
+    assert math.isnan(col[0, 0]) and col[0, 2] == math.inf and rgba[0, 2] == 255 and rgba[0, 0] == 0
+    # a second light: the first (ambient + chaos) plus the full Phong of the second
+    pt[7] = 4.0
+    col, _ = O.shade(pt, cam, lights, 1)
+    lights[1] = lights[0]
+    L1 = lights[1]
+    L1.V = lyap.Vec(0.0, 0.0, -1.0)
+    L1.lightOuterCone, L1.lightInnerCone = 0.5, 0.9
+    L1.chaosColor = lyap.Color(0.0, 0.0, 0.0, 0.0)
+    col2, _ = O.shade(pt, cam, lights, 2)
+    np.testing.assert_allclose(col2[0], col[0] + full_phong(L1) + amb(L1), rtol=2e-6)
+
+
+@pytest.mark.parametrize("lam_mode", [O.F64, O.F32W])
+def test_walk_reverses_and_halves(lam_mode):
+    """The reference's refinement walk on a ray along -x through [2,3]^3
+    (sequence 'A': lambda = ln|2 - x|, opaque at -0.5 for x <= 2.6065).
+    Camera at x = 4 with S0 = (-1, 0, 0), M = 1/2: V = (-2, 0, 0), the box
+    is t in [1/2, 1], stepMethod 1 with depth 8 gives dt = 1/16 (x steps of
+    1/8). March samples x = 3, 2.875, 2.75, 2.625 (lambda -0.47,
+    transparent), 2.5 (-0.69, opaque). The walk starts at Qdt = -dt/2 =
+    -1/32 (x += 1/16), refine 32 so it stops once |Qdt| < 1/512; each sign
+    change reverses and halves the step:
+      x = 2.5625 (-0.58, opaque), 2.625 (-0.47, transparent: Qdt +1/64),
+      2.59375 (-0.52, opaque: -1/128), 2.609375 (-0.495: +1/256),
+      2.6015625 (-0.508: -1/512), 2.60546875 (-0.5018, opaque, no change),
+      2.609375 (transparent: +1/1024, below 1/512, the walk ends).
+    The hit is x = 2.609375 at t = 0.6953125 (on the transparent side),
+    after 5 + 7 + 6 evaluations."""
+    prm, cam, lights, n = lyap.params_init()
+    prm.settle, prm.accum = 200, 200
+    prm.lMin, prm.lMax = 2.0, 3.0
+    prm.opaqueThreshold, prm.chaosThreshold, prm.nearThreshold = -0.5, 0.5, -1.0
+    prm.stepMethod, prm.depth, prm.jitter, prm.nearMultiplier, prm.refine = 1, 8.0, 0.0, 1.0, 32.0
+    cam.C = lyap.Vec(4.0, 2.5, 2.5)
+    cam.M = 0.5
+    cam.S0 = lyap.Vec(-1.0, 0.0, 0.0)
+    cam.SDX = lyap.Vec(0.0, 0.0, 0.0)
+    cam.SDY = lyap.Vec(0.0, 0.0, 0.0)
+    out = O.render(cam, prm, "A", lights, n, 1, 1, mode=lyap.render_mode(), lam=lam_mode)
+    P = out["points"].reshape(-1)
+    assert P[:3].tolist() == [2.609375, 2.5, 2.5]
+    assert out["depth"].reshape(-1)[0] == 0.6953125
+    assert int(out["evals"].reshape(-1)[0]) == 18
+    assert P[8] == pytest.approx(math.log(0.609375), abs=1e-5)
+
+
+def test_fp32_as_written_differs_from_float_arithmetic():
+    """Sequence AB with x = 3 * 2^-24 and y = 11184812, settle 2, accum 1.
+    Step 1: v = x * 0.5 * (1 - 0.5) = 3 * 2^-26 (exact in every form).
+    Step 2: y * v = 0.5 + 2^-24 exactly. As written (kernel.cu:135) the
+    factor 1.0 - v is double, 1 - 3 * 2^-26, and the product
+    0.5 + 2.5 * 2^-26 - 3 * 2^-50 rounds to the float 0.5 + 2^-24, so the
+    guard (|v - 0.5| < 1e-8) does not fire and lambda is ln|x (1 - 2 v3)|
+    with v3 ~ 0.75 * 2^-24, i.e. ln(3 * 2^-24) to 1e-6. In float arithmetic
+    throughout 1 - v rounds to 1 - 2^-24, the product
+    0.5 + 2^-25 - 2^-48 rounds to 0.5 exactly, and lambda would be 0.
+    FP64 agrees with the as-written form here."""
+    x, y = 3.0 * 2.0 ** -24, 11184812.0
+    pts = np.array([[x, y, 0.0]], np.float32)
+    f32w = float(O.lyap_points(pts, 2.1, 2, 1, "AB", mode=O.F32W)[0])
+    f64 = float(O.lyap_points(pts, 2.1, 2, 1, "AB", mode=O.F64)[0])
+    expect = math.log(3.0 * 2.0 ** -24)
+    assert f32w == pytest.approx(expect, abs=1e-5) and f64 == pytest.approx(expect, abs=1e-5)
+    # the same two steps in float arithmetic land on 0.5 exactly
+    v1 = np.float32(x) * np.float32(0.25)
+    v2 = np.float32(np.float32(y) * v1) * np.float32(np.float32(1.0) - v1)
+    assert v1 == np.float32(3.0 * 2.0 ** -26) and v2 == np.float32(0.5)
 
 
 @pytest.mark.parametrize("x0,lam_mode", [(2.8, O.F64), (2.8, O.F32W), (3.9, O.F64), (3.9, O.F32W)])

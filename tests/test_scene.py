@@ -76,12 +76,51 @@ def test_cam_recalculate_rotated_and_clamped():
     assert np.allclose([cam.S0.x, cam.S0.y, cam.S0.z], [0.5, -0.5, 1.0], atol=1e-7)
 
 
-def test_cam_normalises_quaternion():
+def test_cam_normalise_quirk():
+    """QUAT::normalize (quat.hpp:147-165) multiplies by |q| outside the 1e-6
+    band around |q|^2 = 1: q = (0,0,0,2) becomes (0,0,0,4); the view axis,
+    normalised afterwards, is still +z."""
     cam = lyap.LyapCam()
     cam.Q.x, cam.Q.y, cam.Q.z, cam.Q.w = 0.0, 0.0, 0.0, 2.0
     cam.M = 0.5
     lyap.cam_recalculate(cam, 4, 4, 1)
-    assert (cam.Q.w, cam.V.z) == (1.0, 1.0)
+    assert (cam.Q.w, cam.V.z) == (4.0, 1.0)
+
+
+def vec(v):
+    return (v.x, v.y, v.z)
+
+
+def test_transform_sums_left_to_right_in_float():
+    """QUAT::transform (quat.hpp:279-284) adds its eight float products in
+    the written order. q = (0, 2^-12, 0, 1), v = (1, 0, 2^-13): the x
+    component is 1 + 2 y w v_z - y^2 v_x = 1 + 2^-24 - 2^-24. In float the
+    first sum 1 + 2^-24 rounds (to even) to 1 and the last term then gives
+    1 - 2^-24; evaluated exactly (or in double, rounded once) it is 1."""
+    q = lyap.Quat(0.0, 2.0 ** -12, 0.0, 1.0)
+    out = lyap.quat_transform(q, (1.0, 0.0, 2.0 ** -13))
+    assert out[0] == 1.0 - 2.0 ** -24
+
+
+@pytest.mark.parametrize("q,axes", [((0.0, 0.0, 0.0, 1.0), (0, 1, 2)), ((0.5, 0.5, 0.5, 0.5), (2, 0, 1))])
+def test_camera_vectors_bitwise(q, axes):
+    """scene_cam_recalculate (scene.cu:31-63) on rotations whose products
+    are exact: the identity, and (1/2, 1/2, 1/2, 1/2), the 120 degree turn
+    about (1,1,1) that maps (a, b, c) to (c, a, b). M = 1/2 and a 4 x 4 image:
+    S0 = Q (-1/2, -1/2, 1), SDX = Q (1/4, 0, 0), SDY = Q (0, 1/4, 0), V = Q z,
+    bit for bit."""
+    cam = lyap.LyapCam()
+    cam.Q.x, cam.Q.y, cam.Q.z, cam.Q.w = q
+    cam.M = 0.5
+    lyap.cam_recalculate(cam, 4, 4, 1)
+
+    def rot(v):
+        return tuple(v[a] for a in axes)
+
+    assert vec(cam.S0) == rot((-0.5, -0.5, 1.0))
+    assert vec(cam.SDX) == rot((0.25, 0.0, 0.0))
+    assert vec(cam.SDY) == rot((0.0, 0.25, 0.0))
+    assert vec(cam.V) == rot((0.0, 0.0, 1.0))
 
 
 def test_quaternions():
