@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define LYAP_ABI_VERSION 1
+#define LYAP_ABI_VERSION 2
 #define LYAP_MAX_SEQ 32     /* longest sequence period accepted */
 #define LYAP_MAX_LIGHTS 16  /* params.hpp:21 */
 
@@ -301,13 +301,20 @@ typedef struct LyapView {
     uint32_t reserved;
 } LyapView;
 
-/* Launch shape of the persistent kernels. Zero fields are filled from the
- * autotuner's table (lyap_autotune) or built-in defaults. */
+/* Launch shape of the persistent kernels (the reference's fixed 16x16 and
+ * 8x8x8 blocks, lyap_interactive.cu:64-66 and lyap_calculate.cu:19-24, are
+ * what it replaces). Zero fields are filled from the autotuner's choice
+ * (lyap_autotune, lyap_autotune_batch) or built-in defaults. */
 typedef struct LyapTuning {
     uint32_t tile_rays;        /* rays handed out per work-queue claim */
     uint32_t waves_per_cu;     /* resident waves requested per CU */
     uint32_t lanes_per_ray;    /* lanes sharing one ray's march */
     uint32_t slots;            /* samples each lane evaluates per round, 1..4 */
+    uint32_t level_budget[3];  /* march samples a ray may take in launch levels 0, 1
+                                  and 2 before it is handed to the next level (0: 24
+                                  at four or more lanes per ray, else 64 / 64 / 128) */
+    uint32_t voxel_tile;       /* voxels per work-queue claim of a volume block,
+                                  rounded up to a multiple of 64 (0: 256) */
 } LyapTuning;
 
 /* Renders n_views views in one persistent launch fed by a device work queue
@@ -315,10 +322,48 @@ typedef struct LyapTuning {
 int lyap_render_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
                       const LyapRenderMode *mode, const LyapTuning *tuning);
 
+/* A block of a volume for lyap_run_batch: z slices [z_begin, z_end) of the
+ * n[0] x n[1] x n[2] grid over [lo, hi) that lyap_calc_volume computes,
+ * lambda written to `out` (device memory; slice z_begin first, x fastest). */
+typedef struct LyapVolumeBlock {
+    LyapParams prm;
+    int32_t seq[LYAP_MAX_SEQ + 1];     /* -1 terminated */
+    float lo[3], hi[3];
+    uint32_t n[3];
+    uint32_t z_begin, z_end;           /* z_end 0: n[2] */
+    float *out;
+} LyapVolumeBlock;
+
+/* Views and volume blocks of different sizes in one persistent launch
+ * sequence (kernel_calc_render and kernel_calc_volume, kernel.hpp:17,19,
+ * in one): the first launch's device work queue deals the views' ray tiles
+ * and then the blocks' voxel tiles (one lane per voxel), so waves whose
+ * rays have ended take voxels; voxels the volume mode's rule lists are
+ * re-evaluated in FP64 by the next launch. Each view gets the bytes
+ * lyap_calc_render gives it and each block the bytes of lyap_calc_volume
+ * over the same grid (the same arithmetic). Blocks are evaluated in the
+ * views' precision: when vol_mode's precision differs they run in the
+ * volume kernels after the views' launches (same results, not fused). */
+int lyap_run_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                   const LyapVolumeBlock *blocks, unsigned n_blocks,
+                   const LyapRenderMode *mode, const LyapVolumeMode *vol_mode,
+                   const LyapTuning *tuning);
+
 /* Times candidate launch shapes on a representative batch and stores the
- * fastest in the context (used when a LyapTuning field is 0). */
+ * fastest in the context (used when a LyapTuning field is 0): lanes per
+ * ray x samples per lane (one lane per ray included) x tile length, then
+ * each level's march-sample budget, then (with volume blocks) the voxel
+ * tile. The results do not depend on the shape. */
 int lyap_autotune(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
                   const LyapRenderMode *mode, LyapTuning *best);
+int lyap_autotune_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                        const LyapVolumeBlock *blocks, unsigned n_blocks,
+                        const LyapRenderMode *mode, const LyapVolumeMode *vol_mode,
+                        LyapTuning *best);
+/* Sets the context's tuned shape (what lyap_autotune stores), or clears it
+ * with NULL; the server's launches use it too. */
+int lyap_set_tuning(lyap_ctx *ctx, const LyapTuning *tuning);
+int lyap_get_tuning(lyap_ctx *ctx, LyapTuning *out);
 
 /* A captured frame: the batch upload and the frame's launches (the march
  * levels and the refinement) recorded once as a HIP graph and replayed. The
@@ -429,15 +474,21 @@ typedef struct LyapRequest {
     double deadline_ms;                /* relative deadline used for ordering */
 } LyapRequest;
 
+/* LyapServerConfig.flags */
+enum { LYAP_SERVER_SEPARATE_VOLUMES = 1 };
+
 typedef struct LyapServerConfig {
     size_t hbm_budget_bytes;           /* cap on resident outputs (0: none) */
-    uint32_t render_slots;             /* render launches in flight, one
-                                          stream each (0: 2) */
-    uint32_t volume_slots;             /* volume launches in flight (0: 1) */
+    uint32_t render_slots;             /* launches in flight, one stream each:
+                                          render_slots + volume_slots in all (0: 2) */
+    uint32_t volume_slots;             /* (0: 1) */
     double slice_ms;                   /* device time aimed at per launch (0: 12) */
     uint32_t chunk_rays;               /* smallest piece of a view a launch
                                           takes, a multiple of 64 (0: 4096) */
-    uint32_t reserved;
+    uint32_t flags;                    /* LYAP_SERVER_SEPARATE_VOLUMES: render slots
+                                          take views only and volume slots run slabs
+                                          in the volume kernels; otherwise every
+                                          launch may carry both kinds */
     LyapRenderMode mode;
     LyapVolumeMode vol_mode;
 } LyapServerConfig;
@@ -466,9 +517,10 @@ typedef struct LyapServerStats {
 /* Continuous batching over one device. A view is cut into 64-ray pieces
  * and a volume into z slabs; each launch takes the pieces of the requests
  * with the earliest deadlines, up to about `slice_ms` of device time, so a
- * request that arrives while others run joins the next launch. Render
- * launches go to their own streams and volume launches to another, so their
- * work overlaps on the device. A request's outputs are admitted against
+ * request that arrives while others run joins the next launch. A launch
+ * carries views and volume slabs together (one persistent launch, as
+ * lyap_run_batch), each slot on its own stream, so the launches' work
+ * overlaps on the device. A request's outputs are admitted against
  * `hbm_budget_bytes` when its first piece is issued and stay resident until
  * lyap_server_release. One host thread drives a server. */
 typedef struct lyap_server lyap_server;

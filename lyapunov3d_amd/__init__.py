@@ -145,6 +145,21 @@ class Tuning(C.Structure):
     _fields_ = [
         ("tile_rays", C.c_uint32), ("waves_per_cu", C.c_uint32),
         ("lanes_per_ray", C.c_uint32), ("slots", C.c_uint32),
+        ("level_budget", C.c_uint32 * 3), ("voxel_tile", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {"tile_rays": self.tile_rays, "waves_per_cu": self.waves_per_cu,
+                "lanes_per_ray": self.lanes_per_ray, "slots": self.slots,
+                "level_budget": list(self.level_budget), "voxel_tile": self.voxel_tile}
+
+
+class VolumeBlock(C.Structure):
+    """LyapVolumeBlock: z slices [z_begin, z_end) of a volume grid."""
+    _fields_ = [
+        ("prm", LyapParams), ("seq", C.c_int32 * (MAX_SEQ + 1)), ("lo", C.c_float * 3),
+        ("hi", C.c_float * 3), ("n", C.c_uint32 * 3), ("z_begin", C.c_uint32), ("z_end", C.c_uint32),
+        ("out", C.c_void_p),
     ]
 
 
@@ -161,7 +176,7 @@ class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_size_t), ("render_slots", C.c_uint32),
         ("volume_slots", C.c_uint32), ("slice_ms", C.c_double), ("chunk_rays", C.c_uint32),
-        ("reserved", C.c_uint32), ("mode", RenderMode), ("vol_mode", VolumeMode),
+        ("flags", C.c_uint32), ("mode", RenderMode), ("vol_mode", VolumeMode),
     ]
 
 
@@ -202,6 +217,7 @@ assert C.sizeof(LyapCam) == 224
 assert C.sizeof(RGBA) == 4
 assert LyapCam.ambient.offset == 128 and LyapCam.chaosColor.offset == 208
 assert C.sizeof(View) == 480 and C.sizeof(Request) == 736
+assert C.sizeof(Tuning) == 32 and C.sizeof(VolumeBlock) == 240
 
 
 # ------------------------------------------------------------- library ----
@@ -250,6 +266,12 @@ _SIGS = {
     "lyap_timing_collect": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "lyap_render_batch": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning)]),
     "lyap_autotune": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning)]),
+    "lyap_run_batch": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(VolumeBlock), C.c_uint,
+                                 C.POINTER(RenderMode), C.POINTER(VolumeMode), C.POINTER(Tuning)]),
+    "lyap_set_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "lyap_get_tuning": (C.c_int, [_P, C.POINTER(Tuning)]),
+    "lyap_autotune_batch": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(VolumeBlock), C.c_uint,
+                                      C.POINTER(RenderMode), C.POINTER(VolumeMode), C.POINTER(Tuning)]),
     "lyap_frame_capture": (C.c_int, [_P, C.POINTER(View), C.c_uint, C.POINTER(RenderMode), C.POINTER(Tuning), C.POINTER(_P)]),
     "lyap_frame_replay": (C.c_int, [_P, _P]),
     "lyap_frame_destroy": (None, [_P]),
@@ -546,6 +568,38 @@ class Context:
         _check(rc, self.h, "lyap_autotune")
         return best
 
+    def run_batch(self, views, blocks, mode: Optional[RenderMode] = None,
+                  vol_mode: Optional[VolumeMode] = None, tuning: Optional[Tuning] = None):
+        """Views and volume blocks in one persistent launch sequence (lyap_run_batch)."""
+        va = (View * max(len(views), 1))(*views)
+        ba = (VolumeBlock * max(len(blocks), 1))(*blocks)
+        rc = self.lib.lyap_run_batch(self.h, va, len(views), ba, len(blocks),
+                                     C.byref(mode) if mode is not None else None,
+                                     C.byref(vol_mode) if vol_mode is not None else None,
+                                     C.byref(tuning) if tuning is not None else None)
+        _check(rc, self.h, "lyap_run_batch")
+
+    def autotune_batch(self, views, blocks, mode: Optional[RenderMode] = None,
+                       vol_mode: Optional[VolumeMode] = None) -> Tuning:
+        va = (View * max(len(views), 1))(*views)
+        ba = (VolumeBlock * max(len(blocks), 1))(*blocks)
+        best = Tuning()
+        rc = self.lib.lyap_autotune_batch(self.h, va, len(views), ba, len(blocks),
+                                          C.byref(mode) if mode is not None else None,
+                                          C.byref(vol_mode) if vol_mode is not None else None, C.byref(best))
+        _check(rc, self.h, "lyap_autotune_batch")
+        return best
+
+    def set_tuning(self, tuning: Optional[Tuning]):
+        """Sets (or with None clears) the context's tuned launch shape."""
+        _check(self.lib.lyap_set_tuning(self.h, C.byref(tuning) if tuning is not None else None), self.h,
+               "lyap_set_tuning")
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        _check(self.lib.lyap_get_tuning(self.h, C.byref(t)), self.h, "lyap_get_tuning")
+        return t
+
     def frame_capture(self, views, mode: Optional[RenderMode] = None, tuning: Optional[Tuning] = None):
         arr = (View * len(views))(*views)
         f = C.c_void_p()
@@ -594,6 +648,25 @@ def make_view(cam: LyapCam, prm: LyapParams, seq, w: int, h: int, rgba: int, poi
     return v
 
 
+def make_block(prm: LyapParams, seq, lo, hi, n, out: int, z_begin: int = 0, z_end: int = 0) -> VolumeBlock:
+    """A VolumeBlock: slices [z_begin, z_end) of the n (or (nx, ny, nz)) grid over
+    [lo, hi); `out` is the device address of slice z_begin's first voxel."""
+    b = VolumeBlock()
+    b.prm = prm
+    s = list(seq_array(seq))
+    if len(s) > MAX_SEQ + 1:
+        raise LyapError(ERR_BAD_SEQUENCE, "sequence longer than %d" % MAX_SEQ)
+    for i, x in enumerate(s):
+        b.seq[i] = x
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+    for k in range(3):
+        b.lo[k], b.hi[k] = lo[k], hi[k]
+    b.n[0], b.n[1], b.n[2] = nx, ny, nz
+    b.z_begin, b.z_end = z_begin, z_end
+    b.out = out or None
+    return b
+
+
 # ------------------------------------------------------------- serving ----
 
 def view_request(view: View, deadline_ms: float = 0.0) -> Request:
@@ -623,10 +696,14 @@ def volume_request(prm: LyapParams, seq, lo, hi, n, out: int = 0, deadline_ms: f
     return r
 
 
+SERVER_SEPARATE_VOLUMES = 1
+
+
 def server_config(hbm_budget_bytes: int = 0, render_slots: int = 0, volume_slots: int = 0,
                   slice_ms: float = 0.0, chunk_rays: int = 0, mode: Optional[RenderMode] = None,
-                  vol_mode: Optional[VolumeMode] = None) -> ServerConfig:
+                  vol_mode: Optional[VolumeMode] = None, flags: int = 0) -> ServerConfig:
     c = ServerConfig()
+    c.flags = flags
     c.hbm_budget_bytes = hbm_budget_bytes
     c.render_slots, c.volume_slots = render_slots, volume_slots
     c.slice_ms, c.chunk_rays = slice_ms, chunk_rays

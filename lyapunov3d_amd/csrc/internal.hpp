@@ -86,9 +86,16 @@ int launch_points_halves(lyap_ctx *ctx, float *lam, float *h1, float *h2, const 
 struct RenderSlot;
 RenderSlot *render_slot_create();
 void render_slot_destroy(lyap_ctx *ctx, RenderSlot *slot);
+/* The slot's counter block: words [2..3] hold the last launch's lambda
+ * evaluations (rays' algorithmic samples plus one per voxel). */
 const uint32_t *render_slot_counters(const RenderSlot *slot);
+/* FP64 re-evaluations of each item of the last launch, kItemCountStride
+ * words apart (views' entries stay 0). */
+constexpr uint32_t kItemCountStride = 32;
+const uint32_t *render_slot_vox_counts(const RenderSlot *slot);
 int render_slot_launch(lyap_ctx *ctx, RenderSlot *slot, const LyapView *views,
                        const uint32_t *ray_begin, const uint32_t *ray_count, unsigned n,
+                       const LyapVolumeBlock *blocks, unsigned nb, const LyapVolumeMode &vmode,
                        const LyapRenderMode &mode, const LyapTuning &tune);
 
 int launch_render(lyap_ctx *ctx, LyapRGBA *rgba, LyapPoint *points,

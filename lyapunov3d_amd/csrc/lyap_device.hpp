@@ -29,10 +29,11 @@
  *              Per accumulated step: one multiply (u^2), one FMA, one
  *              multiply into M.
  *
- *  fp64<L>     FP64 with exactly the reference's operation order
- *              ((r*v)*(1-v) and r - (2r)*v, no fused multiply-add; the whole
- *              library is built with -ffp-contract=off), so the orbit is
- *              bitwise the one an IEEE double CPU computes. The log sum is
+ *  fp64<L>     FP64 map with exactly the reference's operation order
+ *              ((r*v)*(1-v), no fused multiply-add; the whole library is
+ *              built with -ffp-contract=off), so the orbit is bitwise the one
+ *              an IEEE double CPU computes. The derivative r(1 - 2v) only
+ *              feeds the log sum and is formed with one fma; the log sum is
  *              carried as a product with exponent renormalisation every 8
  *              steps, which changes lambda only by rounding (~1e-15).
  *
@@ -274,12 +275,28 @@ struct ProductResult {
     float half1, half2;
 };
 
+/* FP32 maps in u = 1 - 2x with b = r/2. The volume form u' = b u^2 + a with
+ * a = 1 - b (a multiply and an FMA) is what every volume evaluator uses (the
+ * per-period kernels and the window form of volume blocks in the persistent
+ * launch), so they give the same bits for the same voxel; the render's form
+ * u' = 1 + b (u^2 - 1) (two FMAs) needs no a and so half the coefficient
+ * registers. Renormalising the product is exact, so block lengths change no
+ * bit; only the map's rounding does. */
+__device__ __forceinline__ float vmap(float u, float a, float b)
+{
+    return __builtin_fmaf(b, u * u, a);
+}
+
+__device__ __forceinline__ float umap(float u, float b)
+{
+    return __builtin_fmaf(b, __builtin_fmaf(u, u, -1.0f), 1.0f);
+}
+
 /* `n` full periods in blocks of `bp` periods, renormalising after each block
  * and recomputing a block whose product fell below kUnderflowGuard. */
 template <int L>
 __device__ __forceinline__ void product_periods(float &u, ProductState &st,
-                                                const float (&a)[L],
-                                                const float (&b)[L],
+                                                const float (&a)[L], const float (&b)[L],
                                                 uint32_t n, uint32_t bp)
 {
     for (uint32_t p = 0; p < n; p += bp) {
@@ -289,7 +306,7 @@ __device__ __forceinline__ void product_periods(float &u, ProductState &st,
         for (uint32_t k = 0; k < cnt; ++k) {
 #pragma unroll
             for (int j = 0; j < L; ++j) {
-                u = __builtin_fmaf(b[j], u * u, a[j]);
+                u = vmap(u, a[j], b[j]);
                 st.M *= __builtin_fabsf(u);
             }
         }
@@ -299,7 +316,7 @@ __device__ __forceinline__ void product_periods(float &u, ProductState &st,
             for (uint32_t k = 0; k < cnt; ++k) {
 #pragma unroll
                 for (int j = 0; j < L; ++j) {
-                    w = __builtin_fmaf(b[j], w * w, a[j]);
+                    w = vmap(w, a[j], b[j]);
                     st.M *= __builtin_fabsf(w);
                     renorm(st);
                 }
@@ -323,11 +340,11 @@ __device__ __forceinline__ ProductResult lambda_product(float x, float y,
     float u = 0.0f;
     for (uint32_t p = 0; p < e.settle_periods; ++p) {
 #pragma unroll
-        for (int j = 0; j < L; ++j) u = __builtin_fmaf(b[j], u * u, a[j]);
+        for (int j = 0; j < L; ++j) u = vmap(u, a[j], b[j]);
     }
 #pragma unroll
     for (int j = 0; j < L; ++j)
-        if ((uint32_t)j < e.settle_rem) u = __builtin_fmaf(b[j], u * u, a[j]);
+        if ((uint32_t)j < e.settle_rem) u = vmap(u, a[j], b[j]);
 
     ProductResult res;
     /* kernel.cu:138: |x - 0.5| < 1e-8 is |u| < 2e-8 */
@@ -342,7 +359,7 @@ __device__ __forceinline__ ProductResult lambda_product(float x, float y,
 #pragma unroll
     for (int j = 0; j < L; ++j) {
         if ((uint32_t)j >= e.head_begin && (uint32_t)j < e.head_end) {
-            u = __builtin_fmaf(b[j], u * u, a[j]);
+            u = vmap(u, a[j], b[j]);
             st.M *= __builtin_fabsf(u);
             renorm(st);
         }
@@ -351,15 +368,14 @@ __device__ __forceinline__ ProductResult lambda_product(float x, float y,
     if (HALVES) {
         product_periods<L>(u, st, a, b, e.half_periods, e.block_periods);
         st_half = st;
-        product_periods<L>(u, st, a, b, e.acc_periods - e.half_periods,
-                           e.block_periods);
+        product_periods<L>(u, st, a, b, e.acc_periods - e.half_periods, e.block_periods);
     } else {
         product_periods<L>(u, st, a, b, e.acc_periods, e.block_periods);
     }
 #pragma unroll
     for (int j = 0; j < L; ++j) {
         if ((uint32_t)j < e.tail) {
-            u = __builtin_fmaf(b[j], u * u, a[j]);
+            u = vmap(u, a[j], b[j]);
             st.M *= __builtin_fabsf(u);
             renorm(st);
         }
@@ -404,11 +420,11 @@ __device__ __forceinline__ double lambda_fp64(float xf, float yf, float zf,
                                               const EvalDesc &e)
 {
     const double x = xf, y = yf, z = zf, d = e.d;
-    double r[L], r2[L];
+    double r[L], m2r[L];
 #pragma unroll
     for (int j = 0; j < L; ++j) {
         r[j] = pickd(e.letters[j], x, y, z, d);
-        r2[j] = 2.0 * r[j];          /* exact */
+        m2r[j] = -2.0 * r[j];        /* exact */
     }
     double v = 0.5;
     for (uint32_t p = 0; p < e.settle_periods; ++p) {
@@ -427,7 +443,7 @@ __device__ __forceinline__ double lambda_fp64(float xf, float yf, float zf,
     for (int j = 0; j < L; ++j) {
         if ((uint32_t)j >= e.head_begin && (uint32_t)j < e.head_end) {
             v = (r[j] * v) * (1.0 - v);
-            st.M *= __builtin_fabs(r[j] - r2[j] * v);
+            st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
             renorm64(st);
         }
     }
@@ -438,7 +454,7 @@ __device__ __forceinline__ double lambda_fp64(float xf, float yf, float zf,
 #pragma unroll
         for (int j = 0; j < L; ++j) {
             v = (r[j] * v) * (1.0 - v);
-            st.M *= __builtin_fabs(r[j] - r2[j] * v);
+            st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
             if (((j + 1) & 7) == 0) renorm64(st);
         }
         if ((L & 7) != 0) renorm64(st);
@@ -447,7 +463,7 @@ __device__ __forceinline__ double lambda_fp64(float xf, float yf, float zf,
     for (int j = 0; j < L; ++j) {
         if ((uint32_t)j < e.tail) {
             v = (r[j] * v) * (1.0 - v);
-            st.M *= __builtin_fabs(r[j] - r2[j] * v);
+            st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
             renorm64(st);
         }
     }
@@ -459,6 +475,19 @@ __device__ __forceinline__ double lambda_fp64(float xf, float yf, float zf,
 /* Period-agnostic forms: one compiled loop for every period L <= W.         */
 /* ======================================================================== */
 
+/* The coefficients of a window of W slots: b_j = r_j / 2 and, for the
+ * volume form (VOL), a_j = 1 - b_j; step(j, u) is the map at slot j. */
+template <int W, bool VOL>
+struct Coefs {
+    float b[W];
+    float a[VOL ? W : 1];
+    __device__ __forceinline__ float step(int j, float u) const
+    {
+        if constexpr (VOL) return vmap(u, a[j], b[j]);
+        else return umap(u, b[j]);
+    }
+};
+
 /* Steps of a block at positions [lo, hi) (FULL: [0, B), where every block
  * has at least W/2 + 1 steps, so only the upper half is guarded). */
 template <int W, bool FULL>
@@ -468,14 +497,8 @@ __device__ __forceinline__ bool in_block(int j, uint32_t lo, uint32_t hi)
     return (uint32_t)j >= lo && (uint32_t)j < hi;
 }
 
-/* FP32 map in u = 1 - 2x: u' = 1 + b (u^2 - 1), b = r/2. */
-__device__ __forceinline__ float umap(float u, float b)
-{
-    return __builtin_fmaf(b, __builtin_fmaf(u, u, -1.0f), 1.0f);
-}
-
-template <int W>
-__device__ __forceinline__ void load_half_r(float (&b)[W], float x, float y,
+template <int W, bool VOL>
+__device__ __forceinline__ void load_half_r(Coefs<W, VOL> &c, float x, float y,
                                             float z, const EvalDesc &e)
 {
     const float hx = 0.5f * x, hy = 0.5f * y, hz = 0.5f * z, hd = 0.5f * e.d;
@@ -483,21 +506,24 @@ __device__ __forceinline__ void load_half_r(float (&b)[W], float x, float y,
 #pragma unroll
     for (int k = 0; k < 4; ++k) m[k] = e.wmask[k];
 #pragma unroll
-    for (int j = 0; j < W; ++j) b[j] = pick_mask<float>(m, j, hx, hy, hz, hd);
+    for (int j = 0; j < W; ++j) {
+        c.b[j] = pick_mask<float>(m, j, hx, hy, hz, hd);
+        if constexpr (VOL) c.a[j] = 1.0f - c.b[j];
+    }
 }
 
-template <int W, bool FULL>
-__device__ __forceinline__ void settle_block(float &u, const float (&b)[W],
+template <int W, bool VOL, bool FULL>
+__device__ __forceinline__ void settle_block(float &u, const Coefs<W, VOL> &c,
                                              uint32_t lo, uint32_t hi)
 {
 #pragma unroll
     for (int j = 0; j < W; ++j)
-        if (in_block<W, FULL>(j, lo, hi)) u = umap(u, b[j]);
+        if (in_block<W, FULL>(j, lo, hi)) u = c.step(j, u);
 }
 
-template <int W, bool FULL>
+template <int W, bool VOL, bool FULL>
 __device__ __forceinline__ void product_block(float &u, ProductState &st,
-                                              const float (&b)[W], uint32_t lo,
+                                              const Coefs<W, VOL> &c, uint32_t lo,
                                               uint32_t hi)
 {
     const float u0 = u;
@@ -505,7 +531,7 @@ __device__ __forceinline__ void product_block(float &u, ProductState &st,
 #pragma unroll
     for (int j = 0; j < W; ++j) {
         if (in_block<W, FULL>(j, lo, hi)) {
-            u = umap(u, b[j]);
+            u = c.step(j, u);
             st.M *= __builtin_fabsf(u);
         }
     }
@@ -517,7 +543,7 @@ __device__ __forceinline__ void product_block(float &u, ProductState &st,
 #pragma unroll
         for (int j = 0; j < W; ++j) {
             if (in_block<W, FULL>(j, lo, hi)) {
-                w = umap(w, b[j]);
+                w = c.step(j, w);
                 st.M *= __builtin_fabsf(w);
                 renorm(st);
             }
@@ -540,28 +566,28 @@ __device__ __forceinline__ void with_block_len(uint32_t Bv, F &f)
 }
 
 /* n full settle blocks of B steps. */
-template <int W>
+template <int W, bool VOL>
 struct SettleFull {
     float &u;
-    const float (&b)[W];
+    const Coefs<W, VOL> &c;
     uint32_t n;
     template <int B>
     __device__ __forceinline__ void go()
     {
         for (uint32_t k = 0; k < n; ++k) {
 #pragma unroll
-            for (int j = 0; j < B; ++j) u = umap(u, b[j]);
+            for (int j = 0; j < B; ++j) u = c.step(j, u);
         }
     }
 };
 
 /* n full product blocks of B steps, with the same underflow redo as
  * product_block. */
-template <int W>
+template <int W, bool VOL>
 struct ProductFull {
     float &u;
     ProductState &st;
-    const float (&b)[W];
+    const Coefs<W, VOL> &c;
     uint32_t n;
     template <int B>
     __device__ __forceinline__ void go()
@@ -571,7 +597,7 @@ struct ProductFull {
             const ProductState s0 = st;
 #pragma unroll
             for (int j = 0; j < B; ++j) {
-                u = umap(u, b[j]);
+                u = c.step(j, u);
                 st.M *= __builtin_fabsf(u);
             }
             if (st.M < kUnderflowGuard) {
@@ -579,7 +605,7 @@ struct ProductFull {
                 st = s0;
 #pragma unroll
                 for (int j = 0; j < B; ++j) {
-                    w = umap(w, b[j]);
+                    w = c.step(j, w);
                     st.M *= __builtin_fabsf(w);
                     renorm(st);
                 }
@@ -603,16 +629,16 @@ __device__ __forceinline__ void with_len(uint32_t n, F &f)
 
 /* Window positions [LO, HI) of the product form; positions past W/2 are
  * guarded by j < hi when GUARD (they may lie past a block of B < W). */
-template <int W, int LO, int HI, bool GUARD>
+template <int W, bool VOL, int LO, int HI, bool GUARD>
 __device__ __forceinline__ void product_range(float &u, ProductState &st,
-                                              const float (&b)[W], uint32_t hi)
+                                              const Coefs<W, VOL> &c, uint32_t hi)
 {
     const float u0 = u;
     const ProductState s0 = st;
 #pragma unroll
     for (int j = LO; j < HI; ++j) {
         if (!GUARD || j <= W / 2 || (uint32_t)j < hi) {
-            u = umap(u, b[j]);
+            u = c.step(j, u);
             st.M *= __builtin_fabsf(u);
         }
     }
@@ -622,7 +648,7 @@ __device__ __forceinline__ void product_range(float &u, ProductState &st,
 #pragma unroll
         for (int j = LO; j < HI; ++j) {
             if (!GUARD || j <= W / 2 || (uint32_t)j < hi) {
-                w = umap(w, b[j]);
+                w = c.step(j, w);
                 st.M *= __builtin_fabsf(w);
                 renorm(st);
             }
@@ -632,68 +658,69 @@ __device__ __forceinline__ void product_range(float &u, ProductState &st,
 }
 
 /* Settle steps [0, N) / product steps [0, N) / the rest of a block from N. */
-template <int W>
+template <int W, bool VOL>
 struct SettlePrefix {
     float &u;
-    const float (&b)[W];
+    const Coefs<W, VOL> &c;
     template <int N>
     __device__ __forceinline__ void go()
     {
 #pragma unroll
-        for (int j = 0; j < N; ++j) u = umap(u, b[j]);
+        for (int j = 0; j < N; ++j) u = c.step(j, u);
     }
 };
 
-template <int W>
+template <int W, bool VOL>
 struct ProductPrefix {
     float &u;
     ProductState &st;
-    const float (&b)[W];
+    const Coefs<W, VOL> &c;
     template <int N>
-    __device__ __forceinline__ void go() { product_range<W, 0, N, false>(u, st, b, 0); }
+    __device__ __forceinline__ void go() { product_range<W, VOL, 0, N, false>(u, st, c, 0); }
 };
 
-template <int W>
+template <int W, bool VOL>
 struct ProductHead {
     float &u;
     ProductState &st;
-    const float (&b)[W];
+    const Coefs<W, VOL> &c;
     uint32_t hi;
     template <int N>
-    __device__ __forceinline__ void go() { product_range<W, N, W, true>(u, st, b, hi); }
+    __device__ __forceinline__ void go() { product_range<W, VOL, N, W, true>(u, st, c, hi); }
 };
 
-/* FP32 product form, any period L <= W. */
+/* FP32 product form, any period L <= W, with the render's map. */
 template <int W>
 __device__ __forceinline__ float lambda_product_w(float x, float y, float z,
                                                   const EvalDesc &e)
 {
-    float b[W];
-    load_half_r<W>(b, x, y, z, e);
+    constexpr bool VOL = false;
+    Coefs<W, VOL> c;
+    load_half_r<W, VOL>(c, x, y, z, e);
     float u = 0.0f;
     {
-        SettleFull<W> f{u, b, e.set_blocks};
+        SettleFull<W, VOL> f{u, c, e.set_blocks};
         with_block_len<W, W>(e.B, f);
     }
     {
-        SettlePrefix<W> f{u, b};
+        SettlePrefix<W, VOL> f{u, c};
         with_len<W, W - 1>(e.set_rem, f);
     }
     if (__builtin_fabsf(u) < 2e-8f) return 0.0f / (float)e.accum;   /* kernel.cu:138 */
     ProductState st{1.0f, 0};
     if (e.head_hi == e.B) {
         /* the usual head: the rest of the block the settle ended in */
-        ProductHead<W> f{u, st, b, e.B};
+        ProductHead<W, VOL> f{u, st, c, e.B};
         with_len<W, W - 1>(e.head_lo, f);
     } else {
-        product_block<W, false>(u, st, b, e.head_lo, e.head_hi);
+        product_block<W, VOL, false>(u, st, c, e.head_lo, e.head_hi);
     }
     {
-        ProductFull<W> f{u, st, b, e.acc_blocks};
+        ProductFull<W, VOL> f{u, st, c, e.acc_blocks};
         with_block_len<W, W>(e.B, f);
     }
     {
-        ProductPrefix<W> f{u, st, b};
+        ProductPrefix<W, VOL> f{u, st, c};
         with_len<W, W - 1>(e.acc_rem, f);
     }
     float lg[4];
@@ -702,6 +729,85 @@ __device__ __forceinline__ float lambda_product_w(float x, float y, float z,
     lg[2] = __builtin_amdgcn_logf(__builtin_fabsf(z));
     lg[3] = __builtin_amdgcn_logf(__builtin_fabsf(e.d));
     return finish_product(st, lg, e.count, e.accum);
+}
+
+/* n product steps from window position p (p < B): the rest of that block,
+ * whole blocks, then a prefix of one more block. Every part ends with a
+ * renormalisation, which is exact, so where a run is split changes no bit
+ * of the product. */
+template <int W, bool VOL>
+__device__ __forceinline__ void product_run(float &u, ProductState &st, const Coefs<W, VOL> &c,
+                                            uint32_t B, uint32_t p, uint32_t n)
+{
+    if (n == 0) return;
+    if (p != 0) {
+        const uint32_t hi = p + n < B ? p + n : B;
+        if (hi == B) {
+            ProductHead<W, VOL> f{u, st, c, B};
+            with_len<W, W - 1>(p, f);
+        } else {
+            product_block<W, VOL, false>(u, st, c, p, hi);
+        }
+        n -= hi - p;
+    }
+    {
+        ProductFull<W, VOL> f{u, st, c, n / B};
+        with_block_len<W, W>(B, f);
+    }
+    {
+        ProductPrefix<W, VOL> f{u, st, c};
+        with_len<W, W - 1>(n % B, f);
+    }
+}
+
+/* FP32 product form with the volume map, any period L <= W, with the
+ * lambdas of the two parts of the window split after e.half_steps
+ * accumulated steps: the same split, map and products as
+ * lambda_product<L, true>, so the same bits (the products differ only in
+ * where they were renormalised). */
+template <int W>
+__device__ __forceinline__ ProductResult lambda_product_halves_w(float x, float y, float z,
+                                                                 const EvalDesc &e)
+{
+    constexpr bool VOL = true;
+    Coefs<W, VOL> c;
+    load_half_r<W, VOL>(c, x, y, z, e);
+    float u = 0.0f;
+    {
+        SettleFull<W, VOL> f{u, c, e.set_blocks};
+        with_block_len<W, W>(e.B, f);
+    }
+    {
+        SettlePrefix<W, VOL> f{u, c};
+        with_len<W, W - 1>(e.set_rem, f);
+    }
+    ProductResult res;
+    if (__builtin_fabsf(u) < 2e-8f) {                 /* kernel.cu:138 */
+        res.lam = 0.0f / (float)e.accum;
+        res.half1 = res.half2 = res.lam;
+        return res;
+    }
+    ProductState st{1.0f, 0};
+    const uint32_t n1 = e.half_steps;
+    product_run<W, VOL>(u, st, c, e.B, e.set_rem, n1);
+    const ProductState st_half = st;
+    product_run<W, VOL>(u, st, c, e.B, (e.set_rem + n1) % e.B, e.accum - n1);
+    float lg[4];
+    lg[0] = __builtin_amdgcn_logf(__builtin_fabsf(x));
+    lg[1] = __builtin_amdgcn_logf(__builtin_fabsf(y));
+    lg[2] = __builtin_amdgcn_logf(__builtin_fabsf(z));
+    lg[3] = __builtin_amdgcn_logf(__builtin_fabsf(e.d));
+    res.lam = finish_product(st, lg, e.count, e.accum);
+    const uint32_t n2 = e.accum - n1;
+    float c2[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c2[k] = e.count[k] - e.count_half[k];
+    ProductState second;
+    second.E = st.E - st_half.E;
+    second.M = st.M / st_half.M;
+    res.half1 = n1 ? finish_product(st_half, lg, e.count_half, n1) : __builtin_nanf("");
+    res.half2 = n2 ? finish_product(second, lg, c2, n2) : __builtin_nanf("");
+    return res;
 }
 
 /* FP32 x-form with one log2 per accumulated step (the reference method). */

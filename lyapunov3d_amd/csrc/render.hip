@@ -57,7 +57,22 @@ struct ViewDesc {
     uint32_t cont_cap;
     uint32_t refine_base;      /* this view's region of the refinement records */
     uint32_t refine_cap;       /* records per sub-region (kRefineSubs of them) */
+    /* Items of a launch are views or volume blocks (kItemVolume). A volume
+     * block is z slices [vz0, vz0 + n_rays / (vnx * vny)) of an vnx * vny * vnz
+     * grid over [vlo, vlo + vspan), lambda written to vol_out (slice vz0) in
+     * the x-fastest layout of kernel.cu:525; voxels the FP32 pass is unsure of
+     * go to the block's region of the FP64 list. */
+    uint32_t kind;
+    uint32_t tile_len;         /* rays or voxels per level-0 work-queue tile */
+    float *vol_out;
+    uint32_t vnx, vny, vnz, vz0;
+    float vlo[3], vspan[3];
+    float vmargin, vtau, vband;
+    uint32_t vlist_base, vlist_cap;
 };
+
+constexpr uint32_t kItemView = 0;
+constexpr uint32_t kItemVolume = 1;
 
 /* March state of a ray handed to the next launch (96 bytes). */
 struct ContEntry {
@@ -114,6 +129,10 @@ struct RenderArgs {
      * counters per view and sub-region, 32 words apart */
     RefineRec *refine_out;
     uint32_t *refine_counts;
+    /* volume blocks: FP64 re-evaluation list (voxel indices within a block)
+     * and its counters, per item, kHeadStride words apart */
+    uint32_t *vox_list;
+    uint32_t *vox_counts;
 };
 
 namespace {
@@ -370,13 +389,19 @@ __device__ __forceinline__ void write_miss(const ViewDesc &v, const Ray &r)
     o.rgba[r.pix] = z;
 }
 
+/* A NaN normal component (a normal sample's lambda was NaN) as the one
+ * quiet NaN: the refinement launch and refinement in place form the
+ * differences in orders that agree on every number but not on a NaN's sign
+ * bit. */
+__device__ __forceinline__ float canonical(float x) { return __builtin_isnan(x) ? NAN : x; }
+
 __device__ void write_hit(const ViewDesc &v, const Ray &r, F3 N)
 {
     const OutPtrs o = out_ptrs(v);
     if (o.present & 1u) {
         LyapPoint p;
         p.P = {r.P.x, r.P.y, r.P.z};
-        p.N = {N.x, N.y, N.z};
+        p.N = {canonical(N.x), canonical(N.y), canonical(N.z)};
         p.a = r.a;
         p.c = r.c;
         p.l = r.l;
@@ -912,8 +937,9 @@ __device__ __forceinline__ bool tile_lookup(const ViewDesc *__restrict__ views,
         while (v + 1 < a.n_views && views[uniform(v + 1)].tile_begin <= t) ++v;
         view = uniform(v);
         const ViewDesc &d = views[view];
-        first = uniform((t - d.tile_begin) * tile);
-        last = uniform(min(first + tile, d.n_rays));
+        const uint32_t len = uniform(d.tile_len);
+        first = uniform((t - d.tile_begin) * len);
+        last = uniform(min(first + len, d.n_rays));
         return true;
     }
     uint32_t acc = 0;
@@ -960,6 +986,70 @@ __device__ __forceinline__ uint32_t claim_tile(uint32_t *heads, uint32_t n_waves
         ++tried;
     }
     return 0xFFFFFFFFu;
+}
+
+/* Coordinates of voxel i of a volume block: the volume kernels' formula
+ * lo + (span * index) / n per axis (kernel.cu:527-529 for the box [0, 4)). */
+__device__ __forceinline__ void block_voxel(const ViewDesc &v, uint32_t i, float &x, float &y,
+                                            float &z)
+{
+    const uint32_t plane = v.vnx * v.vny;
+    const uint32_t lz = i / plane;
+    const uint32_t rem = i - lz * plane;
+    const uint32_t iy = rem / v.vnx;
+    const uint32_t ix = rem - iy * v.vnx;
+    x = v.vlo[0] + (v.vspan[0] * (float)ix) / (float)v.vnx;
+    y = v.vlo[1] + (v.vspan[1] * (float)iy) / (float)v.vny;
+    z = v.vlo[2] + (v.vspan[2] * (float)(v.vz0 + lz)) / (float)v.vnz;
+}
+
+/* Voxels [first, last) of volume block `item`, one lane per voxel, 64 at a
+ * time, in the precision of the launch. The product form gives the same
+ * bits as the volume kernels (lambda_product<L, true>) and lists the voxels
+ * its rule selects (|lambda| < margin, or window halves that disagree by
+ * more than tau below band) for the FP64 pass, one atomic per wave. Returns
+ * the voxels this lane evaluated. */
+template <int W, int PREC>
+__device__ uint32_t voxel_tile(const ViewDesc &v, uint32_t item, uint32_t first, uint32_t last,
+                               const RenderArgs &a)
+{
+    const uint32_t lane = __lane_id();
+    uint32_t done = 0;
+    for (uint32_t c = first; c < last; c += 64u) {
+        const uint32_t i = c + lane;
+        bool want = false;
+        if (i < last) {
+            float x, y, z;
+            block_voxel(v, i, x, y, z);
+            float lam;
+            if (PREC == LYAP_PREC_PRODUCT) {
+                const ProductResult r = lambda_product_halves_w<W>(x, y, z, v.e);
+                lam = r.lam;
+                want = !(__builtin_fabsf(lam) >= v.vmargin);     /* NaN included */
+                if (v.vtau > 0.0f) {
+                    const float lo_half = r.half1 < r.half2 ? r.half1 : r.half2;
+                    want = want || (lo_half < v.vband && __builtin_fabsf(r.half1 - r.half2) > v.vtau);
+                }
+            } else {
+                lam = eval_lambda<W, PREC>(v.e, x, y, z);
+            }
+            v.vol_out[i] = lam;
+            ++done;
+        }
+        if (PREC == LYAP_PREC_PRODUCT && v.vlist_cap != 0u) {
+            const uint64_t mask = __ballot(want);
+            if (mask != 0) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.vox_counts + item * kHeadStride, (uint32_t)__popcll(mask));
+                base = uniform((uint32_t)__shfl((int)base, 0));
+                if (want) {
+                    const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+                    if (slot < v.vlist_cap) a.vox_list[v.vlist_base + slot] = i;
+                }
+            }
+        }
+    }
+    return done;
 }
 
 /* Three waves per SIMD (at most 168 VGPRs): the 16-slot FP32 kernels fit
@@ -1095,6 +1185,15 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
                     if (sh_count) {
                         shade_list(views[uniform(cur_view)], sh, sh_count);
                         sh_count = 0;
+                    }
+                    if (uniform(views[tv].kind) == kItemVolume) {
+                        /* a tile of a volume block: every lane is idle, so the
+                         * wave evaluates its voxels and claims again */
+                        const unsigned long long s0 = a.wave_log ? clock64() : 0ull;
+                        done_evals += voxel_tile<W, PREC>(views[tv], tv, f, l, a);
+                        if (a.wave_log) c_eval += clock64() - s0;
+                        cur_view = 0xFFFFFFFFu;
+                        continue;
                     }
                     cur_view = tv;
                     pos = f;
@@ -1523,10 +1622,54 @@ __global__ __launch_bounds__(256) void refine_rays(const ViewDesc *__restrict__ 
     for (int off = 32; off > 0; off >>= 1) done += __shfl_down(done, off);
     if (lane == 0 && done) atomicAdd(a.eval_counter, done);
 }
-/* Up to three word arrays cleared by one launch: blockIdx.y picks the array. */
+/* ---- FP64 pass over the volume blocks' lists ---------------------------- */
+
+struct VoxelFp64Args {
+    uint32_t n_items;
+    const uint32_t *list;
+    const uint32_t *counts;            /* per item, kHeadStride words apart */
+};
+
+__device__ __forceinline__ uint32_t listed(const ViewDesc *__restrict__ views,
+                                           const VoxelFp64Args &a, uint32_t p)
+{
+    return uniform(min(a.counts[p * kHeadStride], views[uniform(p)].vlist_cap));
+}
+
+/* Re-evaluates the listed voxels of every volume block in FP64 (the same
+ * lambda_fp64 arithmetic as the volume kernels' FP64 pass), 64 entries of
+ * one block per wave. */
+template <int W>
+__global__ __launch_bounds__(256) void voxel_fp64(const ViewDesc *__restrict__ views, VoxelFp64Args a)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t n_waves = gridDim.x * 4u;
+    const uint32_t wid = uniform(blockIdx.x * 4u + threadIdx.x / 64u);
+    uint32_t total = 0;
+    for (uint32_t p = 0; p < a.n_items; ++p) total += (listed(views, a, p) + 63u) / 64u;
+    for (uint32_t t = wid; t < total; t += n_waves) {
+        uint32_t acc = 0, p = 0, n = 0;
+        for (; p < a.n_items; ++p) {
+            n = listed(views, a, p);
+            const uint32_t nt = (n + 63u) / 64u;
+            if (t < acc + nt) break;
+            acc += nt;
+        }
+        const ViewDesc &v = views[uniform(p)];
+        const uint32_t j = (t - acc) * 64u + lane;
+        if (j < n) {
+            const uint32_t i = a.list[v.vlist_base + j];
+            float x, y, z;
+            block_voxel(v, i, x, y, z);
+            v.vol_out[i] = (float)lambda_fp64_w<W>(x, y, z, v.e);
+        }
+    }
+}
+
+/* Up to four word arrays cleared by one launch: blockIdx.y picks the array. */
 struct ZeroRegions {
-    uint32_t *p[3];
-    uint32_t n[3];
+    uint32_t *p[4];
+    uint32_t n[4];
 };
 
 __global__ void zero_regions(ZeroRegions z)
@@ -1565,6 +1708,35 @@ static void fill_view(ViewDesc &v, const EvalDesc &e, const LyapCam &cam,
     v.normal_h = mode.normal_h;
     v.margin = mode.fp64_margin;
     v.max_samples = 1u << 20;
+    v.kind = kItemView;
+}
+
+/* Fills a volume-block item: z slices [z_begin, z_end) of the nx*ny*nz grid
+ * over [lo, hi) with output `out` (slice z_begin), under `mode`'s rule. */
+static void fill_block(ViewDesc &v, const EvalDesc &e, const LyapParams &prm, const float lo[3],
+                       const float hi[3], const uint32_t n[3], uint32_t z_begin, uint32_t z_end,
+                       float *out, const LyapVolumeMode &mode)
+{
+    std::memset(&v, 0, sizeof(v));
+    v.kind = kItemVolume;
+    v.e = e;
+    v.prm = prm;
+    v.vol_out = out;
+    v.vnx = n[0];
+    v.vny = n[1];
+    v.vnz = n[2];
+    v.vz0 = z_begin;
+    for (int k = 0; k < 3; ++k) {
+        v.vlo[k] = lo[k];
+        v.vspan[k] = hi[k] - lo[k];
+    }
+    v.n_rays = n[0] * n[1] * (z_end - z_begin);
+    const bool reeval = mode.precision == LYAP_PREC_PRODUCT &&
+                        (mode.fp64_margin > 0.0f || mode.converge_tau > 0.0f);
+    v.vmargin = reeval ? mode.fp64_margin : 0.0f;
+    v.vtau = reeval ? mode.converge_tau : 0.0f;
+    v.vband = mode.converge_band;
+    v.vlist_cap = reeval ? v.n_rays : 0u;
 }
 
 /* ---- launch planning ----------------------------------------------------- */
@@ -1587,8 +1759,14 @@ static uint32_t default_lanes_per_ray(uint32_t resident_lanes, uint64_t rays)
  * claims are many times the number of waves and the queue balances. */
 struct LaunchPlan {
     uint32_t blocks, lanes_per_ray, tile_rays, resident_lanes, slots;
+    uint32_t voxel_tile;          /* voxels per claim of a volume block */
+    uint32_t budget[3];           /* level budgets chosen by the tuning, 0 = built-in */
     bool direct;
 };
+
+/* Voxels per work-queue claim of a volume block unless tuned: four waves'
+ * worth, so claims are cheap next to the evaluations they hand out. */
+static const uint32_t kDefaultVoxelTile = 256u;
 
 /* Samples each lane evaluates per round unless tuned. More than one halves
  * the lanes a ray's control work is replicated on, but measured on the
@@ -1605,7 +1783,8 @@ static const uint32_t kLevelBudget[4] = {64u, 64u, 128u, 0u};
  * the launch is large and handing over costs more than it saves. */
 static uint32_t level_budget(uint32_t level, const LaunchPlan &plan)
 {
-    if (plan.direct) return 0u;
+    if (plan.direct || level >= 3) return 0u;
+    if (plan.budget[level]) return plan.budget[level];
     if (level == 0 && plan.lanes_per_ray >= 4u) return 24u;
     return kLevelBudget[level];
 }
@@ -1665,6 +1844,8 @@ static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, const LyapTuni
     if (p.lanes_per_ray * p.slots > 64u) p.lanes_per_ray = 64u / p.slots;
     p.tile_rays = tune.tile_rays ? tune.tile_rays : 64u / p.lanes_per_ray;
     if (p.tile_rays == 0) p.tile_rays = 1;
+    p.voxel_tile = tune.voxel_tile ? (tune.voxel_tile + 63u) / 64u * 64u : kDefaultVoxelTile;
+    for (int k = 0; k < 3; ++k) p.budget[k] = tune.level_budget[k];
     if (direct) {
         p.lanes_per_ray = 1;
         p.slots = 1;
@@ -1701,6 +1882,10 @@ struct RenderResources {
     size_t refine_buf_bytes = 0;
     void *refine_counts = nullptr;   /* per view and sub-region, one 128 B line each */
     size_t refine_counts_bytes = 0;
+    void *vox_list = nullptr;        /* volume blocks' FP64 re-evaluation lists */
+    size_t vox_list_bytes = 0;
+    void *vox_counts = nullptr;      /* per item, one 128 B line each */
+    size_t vox_counts_bytes = 0;
 };
 
 static int ensure_buffer(lyap_ctx *ctx, void **buf, size_t *cap, size_t bytes, bool may_sync)
@@ -1723,6 +1908,8 @@ static void release_resources(lyap_ctx *ctx, RenderResources &r)
     ctx_free(ctx, r.cont_counts);
     ctx_free(ctx, r.refine_buf);
     ctx_free(ctx, r.refine_counts);
+    ctx_free(ctx, r.vox_list);
+    ctx_free(ctx, r.vox_counts);
     ctx_free(ctx, r.counters);
     for (int k = 0; k < 2; ++k) {
         if (r.staging[k]) (void)hipHostFree(r.staging[k]);
@@ -1736,7 +1923,9 @@ static void release_resources(lyap_ctx *ctx, RenderResources &r)
  * reference's launch shape keeps every ray on its lane). */
 static bool parks(const LaunchPlan &plan, const ViewDesc *views)
 {
-    return !plan.direct && views[0].refine == LYAP_REFINE_BISECT && views[0].bisect_steps > 0;
+    /* views come before volume blocks in a launch's table */
+    return !plan.direct && views[0].kind == kItemView && views[0].refine == LYAP_REFINE_BISECT &&
+           views[0].bisect_steps > 0;
 }
 
 /* Prepares `views` for a launch: tile ranges, continuation regions, the
@@ -1747,10 +1936,24 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
                          bool may_sync, uint32_t *total_tiles)
 {
     uint32_t tiles = 0;
-    uint64_t cap_total = 0, refine_total = 0;
+    uint64_t cap_total = 0, refine_total = 0, list_total = 0;
     for (unsigned i = 0; i < n_views; ++i) {
         if (views[i].e.W != W) rebuild_blocks(&views[i].e, W);
         views[i].tile_begin = tiles;
+        if (views[i].kind == kItemVolume) {
+            /* a volume block: voxel tiles, its list region, no ray buffers */
+            views[i].tile_len = plan.voxel_tile;
+            views[i].n_tiles = (views[i].n_rays + plan.voxel_tile - 1) / plan.voxel_tile;
+            tiles += views[i].n_tiles;
+            views[i].cont_base = (uint32_t)cap_total;
+            views[i].cont_cap = 0;
+            views[i].refine_base = (uint32_t)refine_total;
+            views[i].refine_cap = 0;
+            views[i].vlist_base = (uint32_t)list_total;
+            list_total += views[i].vlist_cap;
+            continue;
+        }
+        views[i].tile_len = plan.tile_rays;
         views[i].n_tiles = (views[i].n_rays + plan.tile_rays - 1) / plan.tile_rays;
         tiles += views[i].n_tiles;
         /* continuation region: a quarter of the view's rays, at least 256 */
@@ -1768,8 +1971,8 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
         views[i].refine_cap = rcap;
         refine_total += (uint64_t)rcap * kRefineSubs;
     }
-    if (cap_total > 0xFFFFFFFFull || refine_total > 0xFFFFFFFFull)
-        return fail(ctx, LYAP_ERR_UNSUPPORTED, "too many rays in one launch");
+    if (cap_total > 0xFFFFFFFFull || refine_total > 0xFFFFFFFFull || list_total > 0xFFFFFFFFull)
+        return fail(ctx, LYAP_ERR_UNSUPPORTED, "too many rays or voxels in one launch");
     *total_tiles = tiles;
     const size_t table = sizeof(ViewDesc) * n_views;
     int rc = ensure_buffer(ctx, &res.cont_buf, &res.cont_buf_bytes, 2 * cap_total * sizeof(ContEntry), may_sync);
@@ -1784,6 +1987,13 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
                            (size_t)n_views * kRefineSubs * kHeadStride * sizeof(uint32_t), may_sync);
         if (rc) return rc;
     }
+    if (list_total > 0) {
+        rc = ensure_buffer(ctx, &res.vox_list, &res.vox_list_bytes, list_total * sizeof(uint32_t), may_sync);
+        if (rc) return rc;
+    }
+    rc = ensure_buffer(ctx, &res.vox_counts, &res.vox_counts_bytes,
+                       (size_t)n_views * kHeadStride * sizeof(uint32_t), may_sync);
+    if (rc) return rc;
     rc = ensure_buffer(ctx, &res.view_table, &res.view_table_bytes, table, may_sync);
     if (rc) return rc;
     if (!res.counters) {
@@ -1828,12 +2038,24 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
     z.n[1] = 3 * n_views;
     z.p[2] = park ? (uint32_t *)res.refine_counts : nullptr;
     z.n[2] = park ? n_views * kRefineSubs * kHeadStride : 0u;
-    const uint32_t zmax = std::max(z.n[0], std::max(z.n[1], z.n[2]));
-    hipLaunchKernelGGL(zero_regions, dim3((zmax + 255u) / 256u, 3), dim3(256), 0, ctx->stream, z);
+    z.p[3] = (uint32_t *)res.vox_counts;
+    z.n[3] = n_views * kHeadStride;
+    const uint32_t zmax = std::max(std::max(z.n[0], z.n[1]), std::max(z.n[2], z.n[3]));
+    hipLaunchKernelGGL(zero_regions, dim3((zmax + 255u) / 256u, 4), dim3(256), 0, ctx->stream, z);
     LYAP_HIP(ctx, hipGetLastError());
     int rc = LYAP_OK;
-    uint64_t cap_total = 0;
-    for (unsigned i = 0; i < n_views; ++i) cap_total += views[i].cont_cap;
+    uint64_t cap_total = 0, list_total = 0, lanes0 = 0;
+    bool has_rays = false;
+    for (unsigned i = 0; i < n_views; ++i) {
+        cap_total += views[i].cont_cap;
+        list_total += views[i].vlist_cap;
+        if (views[i].kind == kItemView) {
+            has_rays = true;
+            lanes0 += (uint64_t)views[i].n_tiles * plan.tile_rays * plan.lanes_per_ray;
+        } else {
+            lanes0 += (uint64_t)views[i].n_tiles * 64u;
+        }
+    }
     ContEntry *bufA = (ContEntry *)res.cont_buf;
     ContEntry *bufB = bufA + cap_total;
     uint32_t *counts = (uint32_t *)res.cont_counts;
@@ -1841,11 +2063,13 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
     for (unsigned i = 0; i < n_views; ++i)
         if (views[i].margin > 0.0f && views[i].refine == LYAP_REFINE_BISECT) margin = true;
     const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
-    const uint64_t needed0 = ((uint64_t)total_tiles * plan.tile_rays * plan.lanes_per_ray + 255u) / 256u;
+    (void)total_tiles;
+    const uint64_t needed0 = (lanes0 + 255u) / 256u;
     /* level l reads the rays level l-1 handed over (buffers alternate) */
     ContEntry *out_buf[4] = {bufA, bufB, bufA, nullptr};
     const ContEntry *in_buf[4] = {nullptr, bufA, bufB, bufA};
-    const uint32_t levels = plan.direct ? 1u : 4u;
+    /* volume blocks finish at level 0; later levels carry rays only */
+    const uint32_t levels = (plan.direct || !has_rays) ? 1u : 4u;
     for (uint32_t level = 0; level < levels; ++level) {
         RenderArgs a{};
         a.n_views = n_views;
@@ -1866,6 +2090,8 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         a.out_counts = level < 3 ? counts + level * n_views : nullptr;
         a.refine_out = park ? (RefineRec *)res.refine_buf : nullptr;
         a.refine_counts = park ? (uint32_t *)res.refine_counts : nullptr;
+        a.vox_list = (uint32_t *)res.vox_list;
+        a.vox_counts = (uint32_t *)res.vox_counts;
         uint32_t blocks = plan.blocks;
         if (level == 0 && blocks > needed0) blocks = (uint32_t)needed0;
         if (blocks == 0) blocks = 1;
@@ -1890,8 +2116,23 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         LYAP_HIP(ctx, hipGetLastError());
         rc = timing_mark(ctx);
         if (rc) return rc;
+        if (level == 0 && list_total > 0) {
+            /* FP64 re-evaluation of the voxels the blocks listed */
+            VoxelFp64Args va{};
+            va.n_items = n_views;
+            va.list = (const uint32_t *)res.vox_list;
+            va.counts = (const uint32_t *)res.vox_counts;
+            const uint32_t vb = (uint32_t)std::min<uint64_t>(cus * 8u, (list_total + 255u) / 256u);
+            if (W == 16)
+                hipLaunchKernelGGL(voxel_fp64<16>, dim3(vb), dim3(256), 0, ctx->stream,
+                                   (const ViewDesc *)res.view_table, va);
+            else
+                hipLaunchKernelGGL(voxel_fp64<32>, dim3(vb), dim3(256), 0, ctx->stream,
+                                   (const ViewDesc *)res.view_table, va);
+            LYAP_HIP(ctx, hipGetLastError());
+        }
     }
-    if (park) {
+    if (park && has_rays) {
         RefineArgs ra{};
         ra.n_views = n_views;
         ra.recs = (const RefineRec *)res.refine_buf;
@@ -2001,11 +2242,37 @@ static int build_batch(lyap_ctx *ctx, const LyapView *in, unsigned n, const Lyap
     return LYAP_OK;
 }
 
+/* Appends the validated volume blocks to a launch's items (after its views)
+ * and widens the coefficient window if a block's sequence needs it. */
+static int append_blocks(lyap_ctx *ctx, const LyapVolumeBlock *in, unsigned n,
+                         const LyapVolumeMode &vmode, std::vector<ViewDesc> &items, uint32_t &W)
+{
+    for (unsigned i = 0; i < n; ++i) {
+        const LyapVolumeBlock &b = in[i];
+        const uint32_t z1 = b.z_end ? b.z_end : b.n[2];
+        if (b.n[0] == 0 || b.n[1] == 0 || b.n[2] == 0 || b.z_begin >= z1 || z1 > b.n[2])
+            return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "volume block %u: empty grid or bad z range", i);
+        if ((uint64_t)b.n[0] * b.n[1] * (z1 - b.z_begin) > 0xFFFFFFFFull)
+            return fail(ctx, LYAP_ERR_UNSUPPORTED, "volume block %u: more than 2^32-1 voxels", i);
+        if (!b.out) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "volume block %u: null output", i);
+        EvalDesc e;
+        if (!make_eval_desc(&e, b.prm, b.seq, vmode.renorm))
+            return fail(ctx, LYAP_ERR_BAD_SEQUENCE, "volume block %u: sequence must hold 1..%d letter "
+                        "indices 0..3 before the -1", i, LYAP_MAX_SEQ);
+        ViewDesc v;
+        fill_block(v, e, b.prm, b.lo, b.hi, b.n, b.z_begin, z1, b.out, vmode);
+        items.push_back(v);
+        if (e.L > 16) W = 32;
+    }
+    return LYAP_OK;
+}
+
 static int render_views(lyap_ctx *ctx, std::vector<ViewDesc> &views, uint32_t W,
                         const LyapRenderMode &mode, const LyapTuning &tune)
 {
     uint64_t rays = 0;
-    for (auto &v : views) rays += v.n_rays;
+    for (auto &v : views)
+        if (v.kind == kItemView) rays += v.n_rays;
     const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
     LyapTuning t = tune;
     if (!t.lanes_per_ray && mode.lanes_per_ray) t.lanes_per_ray = mode.lanes_per_ray;
@@ -2032,6 +2299,9 @@ static LyapTuning merged_tuning(const lyap_ctx *ctx, const LyapTuning *t)
         if (t->waves_per_cu) m.waves_per_cu = t->waves_per_cu;
         if (t->lanes_per_ray) m.lanes_per_ray = t->lanes_per_ray;
         if (t->slots) m.slots = t->slots;
+        for (int k = 0; k < 3; ++k)
+            if (t->level_budget[k]) m.level_budget[k] = t->level_budget[k];
+        if (t->voxel_tile) m.voxel_tile = t->voxel_tile;
     }
     return m;
 }
@@ -2067,14 +2337,28 @@ void render_slot_destroy(lyap_ctx *ctx, RenderSlot *slot)
 
 const uint32_t *render_slot_counters(const RenderSlot *slot) { return slot->res.counters; }
 
+const uint32_t *render_slot_vox_counts(const RenderSlot *slot)
+{
+    return static_cast<const uint32_t *>(slot->res.vox_counts);
+}
+
+static_assert(kItemCountStride == kHeadStride, "per-item counter spacing");
+
 /* Renders rays [ray_begin[i], ray_begin[i] + ray_count[i]) of views[i] for
- * every i, in that order of priority, with the slot's own resources. The
- * caller guarantees the slot's previous launch has finished, so nothing here
- * waits on the device. ray_begin is a multiple of 64 (whole 8x8 blocks). */
+ * every i, in that order of priority, then the volume blocks (items n ..
+ * n + nb - 1 of the launch), with the slot's own resources. The blocks must
+ * use the views' precision (or there are no views). The caller guarantees
+ * the slot's previous launch has finished, so nothing here waits on the
+ * device. ray_begin is a multiple of 64 (whole 8x8 blocks). */
 int render_slot_launch(lyap_ctx *ctx, RenderSlot *slot, const LyapView *in,
                        const uint32_t *ray_begin, const uint32_t *ray_count, unsigned n,
-                       const LyapRenderMode &mode, const LyapTuning &tune)
+                       const LyapVolumeBlock *blocks, unsigned nb, const LyapVolumeMode &vmode,
+                       const LyapRenderMode &mode_in, const LyapTuning &tune)
 {
+    LyapRenderMode mode = mode_in;
+    if (n == 0) mode.precision = vmode.precision;
+    if (nb && mode.precision != vmode.precision)
+        return fail(ctx, LYAP_ERR_UNSUPPORTED, "volume blocks need the views' precision in one launch");
     std::vector<ViewDesc> views(n);
     uint32_t maxL = 1;
     uint64_t rays = 0;
@@ -2086,16 +2370,19 @@ int render_slot_launch(lyap_ctx *ctx, RenderSlot *slot, const LyapView *in,
         rays += ray_count[i];
         if ((uint32_t)views[i].e.L > maxL) maxL = (uint32_t)views[i].e.L;
     }
-    const uint32_t W = maxL <= 16 ? 16u : 32u;
+    uint32_t W = maxL <= 16 ? 16u : 32u;
+    int rc = append_blocks(ctx, blocks, nb, vmode, views, W);
+    if (rc) return rc;
     const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
     LyapTuning t = tune;
     if (!t.lanes_per_ray && mode.lanes_per_ray) t.lanes_per_ray = mode.lanes_per_ray;
     const LaunchPlan plan = plan_launch(ctx, rays, t, W, mode.precision, margin,
                                         mode.schedule == LYAP_SCHEDULE_DIRECT);
     uint32_t total_tiles = 0;
-    int rc = prepare_views(ctx, slot->res, views.data(), n, plan, W, false, &total_tiles);
+    const unsigned items = (unsigned)views.size();
+    rc = prepare_views(ctx, slot->res, views.data(), items, plan, W, false, &total_tiles);
     if (rc) return rc;
-    rc = enqueue_views(ctx, slot->res, views.data(), n, total_tiles, plan, W, mode.precision);
+    rc = enqueue_views(ctx, slot->res, views.data(), items, total_tiles, plan, W, mode.precision);
     if (rc) return rc;
     ctx->last_counters = slot->res.counters;
     return LYAP_OK;
@@ -2208,61 +2495,213 @@ extern "C" int lyap_render_batch(lyap_ctx *ctx, const LyapView *views, unsigned 
     return render_views(ctx, vd, W, mode, merged_tuning(ctx, tuning));
 }
 
+namespace lyap {
+
+/* Validates a volume mode (NULL: plain FP32, no re-evaluation). */
+static int check_volume_mode(lyap_ctx *ctx, const LyapVolumeMode *in, LyapVolumeMode &m)
+{
+    m = LyapVolumeMode{};
+    if (in) m = *in;
+    if (m.precision < LYAP_PREC_PLAIN || m.precision > LYAP_PREC_FP64)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown volume precision %d", m.precision);
+    return LYAP_OK;
+}
+
+/* The items of a mixed launch (views sorted, then blocks) and the render
+ * mode the launch runs in: the views' precision, or the blocks' when there
+ * are no views. `fused` is false when the blocks need another precision than
+ * the views; they are then left out of `items`. */
+static int build_mixed(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                       const LyapVolumeBlock *blocks, unsigned n_blocks, const LyapRenderMode &mode,
+                       const LyapVolumeMode &vmode, std::vector<ViewDesc> &items, uint32_t &W,
+                       LyapRenderMode &launch_mode, bool &fused)
+{
+    int rc = build_batch(ctx, views, n_views, mode, items, W);
+    if (rc) return rc;
+    launch_mode = mode;
+    fused = n_blocks == 0 || n_views == 0 || vmode.precision == mode.precision;
+    if (n_views == 0) launch_mode.precision = vmode.precision;
+    if (fused) return append_blocks(ctx, blocks, n_blocks, vmode, items, W);
+    return LYAP_OK;
+}
+
+/* The blocks of a batch whose precision differs from the views': through
+ * the volume kernels, on the context's stream after the views. */
+static int run_blocks_apart(lyap_ctx *ctx, const LyapVolumeBlock *blocks, unsigned n,
+                            const LyapVolumeMode &vmode)
+{
+    for (unsigned i = 0; i < n; ++i) {
+        const LyapVolumeBlock &b = blocks[i];
+        EvalDesc e;
+        if (!make_eval_desc(&e, b.prm, b.seq, vmode.renorm))
+            return fail(ctx, LYAP_ERR_BAD_SEQUENCE, "volume block %u: bad sequence", i);
+        const uint32_t z1 = b.z_end ? b.z_end : b.n[2];
+        int rc = launch_volume_slab(ctx, b.out, e, b.lo, b.hi, b.n[0], b.n[1], b.n[2], b.z_begin, z1,
+                                    nullptr, 0, vmode);
+        if (rc) return rc;
+    }
+    return LYAP_OK;
+}
+
+} /* namespace lyap */
+
+extern "C" int lyap_run_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                              const LyapVolumeBlock *blocks, unsigned n_blocks,
+                              const LyapRenderMode *mode_in, const LyapVolumeMode *vol_mode_in,
+                              const LyapTuning *tuning)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    if (n_views + n_blocks == 0) return LYAP_OK;
+    if ((n_views && !views) || (n_blocks && !blocks))
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "null views or blocks");
+    LyapRenderMode mode;
+    int rc = check_mode(ctx, mode_in, mode);
+    if (rc) return rc;
+    LyapVolumeMode vmode;
+    rc = check_volume_mode(ctx, vol_mode_in, vmode);
+    if (rc) return rc;
+    std::vector<ViewDesc> items;
+    uint32_t W;
+    LyapRenderMode lmode;
+    bool fused;
+    rc = build_mixed(ctx, views, n_views, blocks, n_blocks, mode, vmode, items, W, lmode, fused);
+    if (rc) return rc;
+    rc = render_views(ctx, items, W, lmode, merged_tuning(ctx, tuning));
+    if (rc) return rc;
+    if (!fused) return run_blocks_apart(ctx, blocks, n_blocks, vmode);
+    return LYAP_OK;
+}
+
+namespace lyap {
+
+/* Milliseconds of `reps` runs of the launch sequence (after one untimed). */
+static int time_items(lyap_ctx *ctx, const std::vector<ViewDesc> &items, uint32_t W,
+                      const LyapRenderMode &mode, const LyapTuning &t, hipEvent_t e0, hipEvent_t e1,
+                      int reps, float &ms)
+{
+    std::vector<ViewDesc> trial = items;
+    int rc = render_views(ctx, trial, W, mode, t);
+    if (rc) return rc;
+    LYAP_HIP(ctx, hipEventRecord(e0, ctx->stream));
+    for (int rep = 0; rep < reps; ++rep) {
+        trial = items;
+        rc = render_views(ctx, trial, W, mode, t);
+        if (rc) return rc;
+    }
+    LYAP_HIP(ctx, hipEventRecord(e1, ctx->stream));
+    LYAP_HIP(ctx, hipEventSynchronize(e1));
+    LYAP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    return LYAP_OK;
+}
+
+/* The staged search of lyap_autotune_batch: launch shape, then the level
+ * budgets one level at a time, then the voxel tile. */
+static int autotune_items(lyap_ctx *ctx, const std::vector<ViewDesc> &items, uint32_t W,
+                          const LyapRenderMode &mode, bool has_rays, bool has_blocks, LyapTuning &best_t)
+{
+    hipEvent_t e0, e1;
+    LYAP_HIP(ctx, hipEventCreate(&e0));
+    LYAP_HIP(ctx, hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    int rc = LYAP_OK;
+    auto trial = [&](const LyapTuning &t) {
+        float ms = 0.f;
+        if (rc == LYAP_OK) rc = time_items(ctx, items, W, mode, t, e0, e1, 2, ms);
+        if (rc == LYAP_OK && ms < best_ms) {
+            best_ms = ms;
+            best_t = t;
+        }
+    };
+    best_t = LyapTuning{};
+    trial(best_t);                                  /* the built-in shape */
+    if (has_rays) {
+        /* samples per ray and round K x S from 1 (one lane per ray) to 16,
+         * tiles of one or two generations of a wave's groups */
+        const uint32_t shapes[][2] = {{1, 1}, {2, 1}, {4, 1}, {8, 1}, {16, 1}, {1, 2}, {2, 2}, {4, 2}, {1, 4}};
+        const LyapTuning base = best_t;
+        for (const auto &sh : shapes) {
+            for (uint32_t g : {1u, 2u}) {
+                LyapTuning t = base;
+                t.lanes_per_ray = sh[0];
+                t.slots = sh[1];
+                t.tile_rays = g * (64u / sh[0]);
+                trial(t);
+            }
+        }
+        /* march-sample budgets of levels 0, 1 and 2, one level at a time */
+        const uint32_t cand[3][4] = {{16, 24, 32, 64}, {32, 48, 64, 96}, {96, 128, 192, 256}};
+        for (int lv = 0; lv < 3 && rc == LYAP_OK; ++lv) {
+            const LyapTuning base2 = best_t;
+            for (uint32_t b : cand[lv]) {
+                LyapTuning t = base2;
+                t.level_budget[lv] = b;
+                trial(t);
+            }
+        }
+    }
+    if (has_blocks) {
+        const LyapTuning base = best_t;
+        for (uint32_t vt : {64u, 128u, 512u, 1024u}) {
+            LyapTuning t = base;
+            t.voxel_tile = vt;
+            trial(t);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+} /* namespace lyap */
+
+extern "C" int lyap_autotune_batch(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
+                                   const LyapVolumeBlock *blocks, unsigned n_blocks,
+                                   const LyapRenderMode *mode_in, const LyapVolumeMode *vol_mode_in,
+                                   LyapTuning *best)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    if (n_views + n_blocks == 0 || (n_views && !views) || (n_blocks && !blocks))
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "nothing to tune on");
+    LyapRenderMode mode;
+    int rc = check_mode(ctx, mode_in, mode);
+    if (rc) return rc;
+    LyapVolumeMode vmode;
+    rc = check_volume_mode(ctx, vol_mode_in, vmode);
+    if (rc) return rc;
+    std::vector<ViewDesc> items;
+    uint32_t W;
+    LyapRenderMode lmode;
+    bool fused;
+    rc = build_mixed(ctx, views, n_views, blocks, n_blocks, mode, vmode, items, W, lmode, fused);
+    if (rc) return rc;
+    LyapTuning t{};
+    rc = autotune_items(ctx, items, W, lmode, n_views > 0, fused && n_blocks > 0, t);
+    if (rc) return rc;
+    ctx->tuned = t;
+    if (best) *best = t;
+    return LYAP_OK;
+}
+
+extern "C" int lyap_set_tuning(lyap_ctx *ctx, const LyapTuning *tuning)
+{
+    if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
+    ctx->tuned = tuning ? *tuning : LyapTuning{};
+    return LYAP_OK;
+}
+
+extern "C" int lyap_get_tuning(lyap_ctx *ctx, LyapTuning *out)
+{
+    if (!ctx || !out) return LYAP_ERR_INVALID_ARGUMENT;
+    *out = ctx->tuned;
+    return LYAP_OK;
+}
+
 extern "C" int lyap_autotune(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
                              const LyapRenderMode *mode_in, LyapTuning *best)
 {
     if (!ctx) return LYAP_ERR_INVALID_ARGUMENT;
     if (n_views == 0 || !views) return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "no views to tune on");
-    LyapRenderMode mode;
-    int rc = check_mode(ctx, mode_in, mode);
-    if (rc) return rc;
-    std::vector<ViewDesc> vd;
-    uint32_t W;
-    rc = build_batch(ctx, views, n_views, mode, vd, W);
-    if (rc) return rc;
-    hipEvent_t e0, e1;
-    LYAP_HIP(ctx, hipEventCreate(&e0));
-    LYAP_HIP(ctx, hipEventCreate(&e1));
-    /* candidates: samples per lane x lanes per ray x tile length in
-     * generations of the wave's groups */
-    const uint32_t slots[] = {1u, 2u, 4u};
-    const uint32_t lanes[] = {1u, 2u, 4u, 8u, 16u};
-    const uint32_t gens[] = {1u, 2u};
-    float best_ms = 1e30f;
-    LyapTuning best_t{};
-    for (uint32_t sl : slots) {
-    for (uint32_t k : lanes) {
-        if (k * sl < 2u || k * sl > 32u) continue;
-        for (uint32_t g : gens) {
-            LyapTuning t{};
-            t.slots = sl;
-            t.lanes_per_ray = k;
-            t.tile_rays = g * (64u / k);
-            std::vector<ViewDesc> trial = vd;
-            rc = render_views(ctx, trial, W, mode, t);           /* warm-up */
-            if (rc) break;
-            LYAP_HIP(ctx, hipEventRecord(e0, ctx->stream));
-            for (int rep = 0; rep < 2 && rc == LYAP_OK; ++rep) {
-                trial = vd;
-                rc = render_views(ctx, trial, W, mode, t);
-            }
-            if (rc) break;
-            LYAP_HIP(ctx, hipEventRecord(e1, ctx->stream));
-            LYAP_HIP(ctx, hipEventSynchronize(e1));
-            float ms = 0.f;
-            LYAP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best_ms) { best_ms = ms; best_t = t; }
-        }
-        if (rc) break;
-    }
-        if (rc) break;
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    ctx->tuned = best_t;
-    if (best) *best = best_t;
-    return LYAP_OK;
+    return lyap_autotune_batch(ctx, views, n_views, nullptr, 0, mode_in, nullptr, best);
 }
 
 extern "C" int lyap_frame_capture(lyap_ctx *ctx, const LyapView *views, unsigned n_views,

@@ -7,10 +7,14 @@
  *
  *  - a view is a range of 64-ray blocks (8x8 pixels), cut into pieces of a
  *    multiple of 64 rays; a volume is a range of z slices;
- *  - a free render slot takes the pending pieces with the earliest deadlines
- *    up to about slice_ms of estimated device time and runs them as one
- *    persistent launch (the batched path: lane groups, continuation levels);
- *    a free volume slot takes z slabs the same way;
+ *  - a free slot takes the pending pieces of both kinds with the earliest
+ *    deadlines, up to about slice_ms of estimated device time, and runs them
+ *    as one persistent launch: the work queue of its first launch deals the
+ *    views' ray tiles and then the slabs' voxel tiles (lyap_run_batch's
+ *    path), so the waves whose rays end take voxels. With
+ *    LYAP_SERVER_SEPARATE_VOLUMES (or when the volume precision differs from
+ *    the views') render slots take views only and volume slots run slabs in
+ *    the volume kernels;
  *  - every slot owns a context of its own (stream, launch buffers, counters),
  *    so launches of different slots overlap on the device and a launch that
  *    is finishing its long rays shares the CUs with the next one;
@@ -63,19 +67,25 @@ constexpr size_t kMaxSlabs = 16;
 struct Piece {
     uint64_t ticket;
     uint32_t begin, count;
+    int kind;                          /* LYAP_REQ_* */
+    uint32_t item;                     /* volume: index among the launch's slabs */
 };
+
+enum SlotKind { SLOT_MIXED = 0, SLOT_RENDER = 1, SLOT_VOLUME = 2 };
 
 struct Slot {
     lyap_ctx *ctx = nullptr;
-    bool render = true;
+    int kind = SLOT_MIXED;
     RenderSlot *rs = nullptr;
     hipEvent_t start = nullptr, end = nullptr;
     bool busy = false;
     std::vector<Piece> pieces;
-    double est_units = 0;
-    /* pinned: the render launch's evaluations, or each slab's re-evaluations */
-    unsigned long long *host_count = nullptr;
-    uint64_t work = 0;                          /* rays or voxels */
+    double est_view_ms = 0, est_vol_ms = 0;    /* estimated device time of each part */
+    /* pinned: the launch's lambda evaluations (rays' samples, one per voxel) */
+    unsigned long long *host_evals = nullptr;
+    /* pinned: FP64 re-evaluations of each slab, kItemCountStride words apart */
+    uint32_t *host_reeval = nullptr;
+    uint64_t rays = 0, voxels = 0;              /* work of the launch */
 };
 
 double now_ms(std::chrono::steady_clock::time_point t0)
@@ -229,110 +239,136 @@ void finish(lyap_server *s, Request &r, int status)
 
 bool has_work(const Request &r) { return r.state != FINISHED && !r.cancel && r.next_unit < r.n_units; }
 
-/* Requests of `kind` with pieces left, earliest deadline first. */
+/* Requests of `kind` (-1: both) with pieces left, earliest deadline first. */
 std::vector<Request *> edf_order(lyap_server *s, int kind)
 {
     std::vector<Request *> v;
     for (auto &kv : s->reqs)
-        if (kv.second.rq.kind == kind && has_work(kv.second)) v.push_back(&kv.second);
+        if ((kind < 0 || kv.second.rq.kind == kind) && has_work(kv.second)) v.push_back(&kv.second);
     std::stable_sort(v.begin(), v.end(), [](const Request *a, const Request *b) {
         return a->deadline < b->deadline;
     });
     return v;
 }
 
-int issue_render(lyap_server *s, Slot &sl)
+/* Pixels inside the image among rays [begin, begin + count) of a view (its
+ * 64-ray blocks are 8x8 pixels, clipped at the right and bottom edges). */
+uint64_t piece_pixels(const LyapView &v, uint32_t begin, uint32_t count)
 {
-    const double budget = s->cfg.slice_ms / s->render_ms_per_unit;
+    const uint32_t w = v.width, h = v.height, bx = (w + 7) / 8;
+    uint64_t px = 0;
+    for (uint32_t blk = begin / 64; blk < (begin + count) / 64; ++blk) {
+        const uint32_t x0 = (blk % bx) * 8, y0 = (blk / bx) * 8;
+        px += (uint64_t)std::min(8u, w - x0) * std::min(8u, h - y0);
+    }
+    return px;
+}
+
+/* Takes the earliest-deadline pieces the slot can run, up to about slice_ms
+ * of estimated device time: views' ray ranges (a multiple of chunk_rays, or
+ * what is left), volumes' z slabs (at most kMaxSlabs per launch). */
+int take_pieces(lyap_server *s, Slot &sl, std::vector<LyapView> &views, std::vector<uint32_t> &begin,
+                std::vector<uint32_t> &count, std::vector<LyapVolumeBlock> &blocks)
+{
+    const double budget = s->cfg.slice_ms;
     const uint32_t chunk = s->cfg.chunk_rays;
+    const int want = sl.kind == SLOT_MIXED ? -1 : (sl.kind == SLOT_RENDER ? LYAP_REQ_VIEW : LYAP_REQ_VOLUME);
+    double est = 0;
+    for (Request *r : edf_order(s, want)) {
+        if (est >= budget) break;
+        const bool view = r->rq.kind == LYAP_REQ_VIEW;
+        if (!view && blocks.size() >= kMaxSlabs) continue;
+        if (r->state == QUEUED) {
+            int rc = admit(s, *r);
+            if (rc == LYAP_ERR_BUDGET) continue;
+            if (rc) return rc;
+        }
+        const uint32_t left = r->n_units - r->next_unit;
+        const double rate = view ? s->render_ms_per_unit : s->volume_ms_per_unit;
+        const double room = (budget - est) / (r->unit_cost * rate);
+        uint32_t take = left;
+        if (view) {
+            if (room < (double)left) {
+                take = (uint32_t)std::max<double>(chunk, std::ceil(room / chunk) * chunk);
+                if (take > left) take = left;
+            }
+            views.push_back(r->rq.view);
+            begin.push_back(r->next_unit);
+            count.push_back(take);
+            sl.pieces.push_back({r->ticket, r->next_unit, take, LYAP_REQ_VIEW, 0});
+            sl.rays += piece_pixels(r->rq.view, r->next_unit, take);
+            sl.est_view_ms += take * r->unit_cost * rate;
+        } else {
+            if (room < (double)left) take = std::max<uint32_t>(1u, (uint32_t)room);
+            LyapVolumeBlock b;
+            std::memset(&b, 0, sizeof(b));
+            b.prm = r->rq.vol_prm;
+            std::memcpy(b.seq, r->rq.vol_seq, sizeof(b.seq));
+            for (int k = 0; k < 3; ++k) {
+                b.lo[k] = r->rq.vol_lo[k];
+                b.hi[k] = r->rq.vol_hi[k];
+                b.n[k] = r->rq.vol_n[k];
+            }
+            b.z_begin = r->next_unit;
+            b.z_end = r->next_unit + take;
+            b.out = r->rq.vol_out + (size_t)b.n[0] * b.n[1] * r->next_unit;
+            sl.pieces.push_back({r->ticket, r->next_unit, take, LYAP_REQ_VOLUME, (uint32_t)blocks.size()});
+            blocks.push_back(b);
+            sl.voxels += (uint64_t)b.n[0] * b.n[1] * take;
+            sl.est_vol_ms += take * r->unit_cost * rate;
+        }
+        r->next_unit += take;
+        r->outstanding++;
+        est = sl.est_view_ms + sl.est_vol_ms;
+    }
+    return LYAP_OK;
+}
+
+int issue(lyap_server *s, Slot &sl)
+{
     std::vector<LyapView> views;
     std::vector<uint32_t> begin, count;
+    std::vector<LyapVolumeBlock> blocks;
     sl.pieces.clear();
-    sl.est_units = 0;
-    sl.work = 0;
-    for (Request *r : edf_order(s, LYAP_REQ_VIEW)) {
-        if (sl.est_units >= budget) break;
-        if (r->state == QUEUED) {
-            int rc = admit(s, *r);
-            if (rc == LYAP_ERR_BUDGET) continue;
-            if (rc) return rc;
-        }
-        const uint32_t left = r->n_units - r->next_unit;
-        double room = (budget - sl.est_units) / r->unit_cost;
-        uint32_t take = left;
-        if (room < (double)left) {
-            take = (uint32_t)std::max<double>(chunk, std::ceil(room / chunk) * chunk);
-            if (take > left) take = left;
-        }
-        views.push_back(r->rq.view);
-        begin.push_back(r->next_unit);
-        count.push_back(take);
-        sl.pieces.push_back({r->ticket, r->next_unit, take});
-        r->next_unit += take;
-        r->outstanding++;
-        sl.est_units += take * r->unit_cost;
-        uint64_t px = 0;   /* pixels inside the image among the piece's rays */
-        {
-            const uint32_t w = r->rq.view.width, h = r->rq.view.height, bx = (w + 7) / 8;
-            for (uint32_t blk = begin.back() / 64; blk < (begin.back() + take) / 64; ++blk) {
-                const uint32_t x0 = (blk % bx) * 8, y0 = (blk / bx) * 8;
-                px += (uint64_t)std::min(8u, w - x0) * std::min(8u, h - y0);
-            }
-        }
-        sl.work += px;
-    }
-    if (views.empty()) return LYAP_OK;
-    LYAP_HIP(sl.ctx, hipEventRecord(sl.start, sl.ctx->stream));
-    int rc = render_slot_launch(sl.ctx, sl.rs, views.data(), begin.data(), count.data(),
-                                (unsigned)views.size(), s->cfg.mode, s->ctx->tuned);
+    sl.est_view_ms = sl.est_vol_ms = 0;
+    sl.rays = sl.voxels = 0;
+    int rc = take_pieces(s, sl, views, begin, count, blocks);
     if (rc) return rc;
-    LYAP_HIP(sl.ctx, hipMemcpyAsync(sl.host_count, render_slot_counters(sl.rs) + 2, 8,
-                                    hipMemcpyDeviceToHost, sl.ctx->stream));
-    LYAP_HIP(sl.ctx, hipEventRecord(sl.end, sl.ctx->stream));
-    sl.busy = true;
-    s->stats.launches++;
-    return LYAP_OK;
-}
-
-int issue_volume(lyap_server *s, Slot &sl)
-{
-    const double budget = s->cfg.slice_ms / s->volume_ms_per_unit;
-    sl.pieces.clear();
-    sl.est_units = 0;
-    sl.work = 0;
-    for (Request *r : edf_order(s, LYAP_REQ_VOLUME)) {
-        if (sl.est_units >= budget || sl.pieces.size() >= kMaxSlabs) break;
-        if (r->state == QUEUED) {
-            int rc = admit(s, *r);
-            if (rc == LYAP_ERR_BUDGET) continue;
-            if (rc) return rc;
-        }
-        const uint32_t left = r->n_units - r->next_unit;
-        const double room = (budget - sl.est_units) / r->unit_cost;
-        const uint32_t take = room < (double)left ? std::max<uint32_t>(1u, (uint32_t)room) : left;
-        const uint32_t nx = r->rq.vol_n[0], ny = r->rq.vol_n[1], nz = r->rq.vol_n[2];
-        if (sl.pieces.empty()) LYAP_HIP(sl.ctx, hipEventRecord(sl.start, sl.ctx->stream));
-        float *out = r->rq.vol_out + (size_t)nx * ny * r->next_unit;
-        int rc = launch_volume_slab(sl.ctx, out, r->vol_e, r->rq.vol_lo, r->rq.vol_hi, nx, ny, nz,
-                                    r->next_unit, r->next_unit + take, nullptr, 0, s->cfg.vol_mode);
-        if (rc) return rc;
-        /* the slab's re-evaluation count (the counter restarts per slab) */
-        LYAP_HIP(sl.ctx, hipMemcpyAsync(sl.host_count + sl.pieces.size(), sl.ctx->counters, 4,
-                                        hipMemcpyDeviceToHost, sl.ctx->stream));
-        sl.pieces.push_back({r->ticket, r->next_unit, take});
-        r->next_unit += take;
-        r->outstanding++;
-        sl.est_units += take * r->unit_cost;
-        sl.work += (uint64_t)nx * ny * take;
-    }
     if (sl.pieces.empty()) return LYAP_OK;
+    LYAP_HIP(sl.ctx, hipEventRecord(sl.start, sl.ctx->stream));
+    if (sl.kind == SLOT_VOLUME) {
+        /* separate volume slot: the volume kernels, slab by slab; each slab's
+         * re-evaluation count (the counter restarts per slab) */
+        for (size_t i = 0; i < blocks.size(); ++i) {
+            const LyapVolumeBlock &b = blocks[i];
+            const Request &r = s->reqs.at(sl.pieces[i].ticket);
+            rc = launch_volume_slab(sl.ctx, b.out, r.vol_e, b.lo, b.hi, b.n[0], b.n[1], b.n[2], b.z_begin,
+                                    b.z_end, nullptr, 0, s->cfg.vol_mode);
+            if (rc) return rc;
+            LYAP_HIP(sl.ctx, hipMemcpyAsync(sl.host_reeval + i * kItemCountStride, sl.ctx->counters, 4,
+                                            hipMemcpyDeviceToHost, sl.ctx->stream));
+        }
+        *sl.host_evals = sl.voxels;
+    } else {
+        rc = render_slot_launch(sl.ctx, sl.rs, views.data(), begin.data(), count.data(),
+                                (unsigned)views.size(), blocks.data(), (unsigned)blocks.size(),
+                                s->cfg.vol_mode, s->cfg.mode, s->ctx->tuned);
+        if (rc) return rc;
+        LYAP_HIP(sl.ctx, hipMemcpyAsync(sl.host_evals, render_slot_counters(sl.rs) + 2, 8,
+                                        hipMemcpyDeviceToHost, sl.ctx->stream));
+        if (!blocks.empty())
+            LYAP_HIP(sl.ctx, hipMemcpyAsync(sl.host_reeval,
+                                            render_slot_vox_counts(sl.rs) + views.size() * kItemCountStride,
+                                            blocks.size() * kItemCountStride * sizeof(uint32_t),
+                                            hipMemcpyDeviceToHost, sl.ctx->stream));
+    }
     LYAP_HIP(sl.ctx, hipEventRecord(sl.end, sl.ctx->stream));
     sl.busy = true;
     s->stats.launches++;
     return LYAP_OK;
 }
 
-/* Marks the pieces of a finished launch done; returns true when it was. */
+/* Marks the pieces of a finished launch done. */
 int retire(lyap_server *s, Slot &sl, bool *changed)
 {
     if (!sl.busy) return LYAP_OK;
@@ -343,38 +379,33 @@ int retire(lyap_server *s, Slot &sl, bool *changed)
     const int status = q == hipSuccess ? LYAP_OK : LYAP_ERR_DEVICE;
     if (status != LYAP_OK) s->ctx->last_error = hipGetErrorString(q);
     float ms = 0.0f;
-    if (status == LYAP_OK && hipEventElapsedTime(&ms, sl.start, sl.end) == hipSuccess &&
-        sl.est_units > 0) {
-        /* running measure; launches of several slots overlap, so this is the
-         * per-unit cost under sharing, which is what the next slice sees */
-        double &m = sl.render ? s->render_ms_per_unit : s->volume_ms_per_unit;
-        m = 0.75 * m + 0.25 * ((double)ms / sl.est_units);
+    const double est = sl.est_view_ms + sl.est_vol_ms;
+    if (status == LYAP_OK && hipEventElapsedTime(&ms, sl.start, sl.end) == hipSuccess && est > 0) {
+        /* running measure of milliseconds per iteration unit, scaled by how
+         * far this launch's estimate was off; launches of several slots
+         * overlap, so it is the cost under sharing, which the next slice
+         * sees */
+        const double f = 0.75 + 0.25 * ((double)ms / est);
+        if (sl.est_view_ms > 0) s->render_ms_per_unit *= f;
+        if (sl.est_vol_ms > 0) s->volume_ms_per_unit *= f;
         s->stats.device_ms += ms;
     }
-    if (sl.render) {
-        s->stats.rays += sl.work;
-        if (status == LYAP_OK) s->stats.evals += (uint64_t)sl.host_count[0];
-    } else {
-        s->stats.voxels += sl.work;
-    }
+    s->stats.rays += sl.rays;
+    s->stats.voxels += sl.voxels;
+    if (status == LYAP_OK) s->stats.evals += (uint64_t)*sl.host_evals;
     for (size_t i = 0; i < sl.pieces.size(); ++i) {
         const Piece &p = sl.pieces[i];
-        uint64_t vol_evals = 0;
-        if (!sl.render && status == LYAP_OK) {
-            /* the counter holds 32 bits; a slab has fewer than 2^32 voxels */
-            vol_evals = (uint64_t)sl.host_count[i] & 0xFFFFFFFFull;
-            s->stats.evals += vol_evals;
+        uint64_t reeval = 0;
+        if (p.kind == LYAP_REQ_VOLUME && status == LYAP_OK) {
+            reeval = sl.host_reeval[p.item * kItemCountStride];
+            s->stats.evals += reeval;
         }
         auto it = s->reqs.find(p.ticket);
         if (it == s->reqs.end()) continue;
         Request &r = it->second;
         r.outstanding--;
-        if (!sl.render) {
-            const uint64_t vox = (uint64_t)r.rq.vol_n[0] * r.rq.vol_n[1] * p.count;
-            vol_evals += vox;
-            r.evals += vol_evals;
-            if (status == LYAP_OK) s->stats.evals += vox;
-        }
+        if (p.kind == LYAP_REQ_VOLUME)
+            r.evals += (uint64_t)r.rq.vol_n[0] * r.rq.vol_n[1] * p.count + reeval;
         if (status != LYAP_OK) {
             r.cancel = true;
             if (r.outstanding == 0) finish(s, r, status);
@@ -396,7 +427,7 @@ int step(lyap_server *s, bool *changed)
     }
     for (auto &sl : s->slots) {
         if (sl.busy) continue;
-        int rc = sl.render ? issue_render(s, sl) : issue_volume(s, sl);
+        int rc = issue(s, sl);
         if (rc) {
             /* the pieces taken by the failed launch end their requests */
             for (const Piece &p : sl.pieces) {
@@ -471,6 +502,8 @@ int lyap_server_create(lyap_ctx *ctx, const LyapServerConfig *cfg_in, lyap_serve
     if (cfg_in) cfg = *cfg_in;
     if (cfg.render_slots == 0) cfg.render_slots = 2;
     if (cfg.volume_slots == 0) cfg.volume_slots = 1;
+    if (cfg.flags & ~(uint32_t)LYAP_SERVER_SEPARATE_VOLUMES)
+        return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown server flags 0x%x", cfg.flags);
     if (cfg.render_slots > 4 || cfg.volume_slots > 4)
         return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "at most 4 render and 4 volume slots");
     if (cfg.slice_ms <= 0) cfg.slice_ms = 12.0;
@@ -492,20 +525,26 @@ int lyap_server_create(lyap_ctx *ctx, const LyapServerConfig *cfg_in, lyap_serve
     s->t0 = std::chrono::steady_clock::now();
     int rc = LYAP_OK;
     const unsigned n = cfg.render_slots + cfg.volume_slots;
+    /* one launch carries both kinds only in one precision */
+    const bool separate = (cfg.flags & LYAP_SERVER_SEPARATE_VOLUMES) ||
+                          cfg.mode.precision != cfg.vol_mode.precision;
     s->slots.resize(n);
     for (unsigned i = 0; i < n && rc == LYAP_OK; ++i) {
         Slot &sl = s->slots[i];
-        sl.render = i < cfg.render_slots;
+        sl.kind = !separate ? SLOT_MIXED : (i < cfg.render_slots ? SLOT_RENDER : SLOT_VOLUME);
         rc = lyap_create(&sl.ctx, ctx->device, 0);
         if (rc) break;
+        const size_t reeval_bytes = kMaxSlabs * kItemCountStride * sizeof(uint32_t);
         if (hipEventCreate(&sl.start) != hipSuccess || hipEventCreate(&sl.end) != hipSuccess ||
-            hipHostMalloc((void **)&sl.host_count, kMaxSlabs * sizeof(unsigned long long),
-                          hipHostMallocDefault) != hipSuccess) {
+            hipHostMalloc((void **)&sl.host_evals, sizeof(unsigned long long), hipHostMallocDefault) !=
+                hipSuccess ||
+            hipHostMalloc((void **)&sl.host_reeval, reeval_bytes, hipHostMallocDefault) != hipSuccess) {
             rc = fail(ctx, LYAP_ERR_DEVICE, "server slot set-up failed");
             break;
         }
-        std::memset(sl.host_count, 0, kMaxSlabs * sizeof(unsigned long long));
-        if (sl.render) {
+        *sl.host_evals = 0;
+        std::memset(sl.host_reeval, 0, reeval_bytes);
+        if (sl.kind != SLOT_VOLUME) {
             sl.rs = render_slot_create();
         } else if (cfg.vol_mode.precision == LYAP_PREC_PRODUCT &&
                    (cfg.vol_mode.fp64_margin > 0.0f || cfg.vol_mode.converge_tau > 0.0f)) {
@@ -535,7 +574,8 @@ void lyap_server_destroy(lyap_server *s)
         if (sl.rs) render_slot_destroy(sl.ctx, sl.rs);
         if (sl.start) (void)hipEventDestroy(sl.start);
         if (sl.end) (void)hipEventDestroy(sl.end);
-        if (sl.host_count) (void)hipHostFree(sl.host_count);
+        if (sl.host_evals) (void)hipHostFree(sl.host_evals);
+        if (sl.host_reeval) (void)hipHostFree(sl.host_reeval);
         if (sl.ctx) lyap_destroy(sl.ctx);
     }
     delete s;

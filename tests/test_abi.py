@@ -57,7 +57,9 @@ LAYOUT_C = r"""
 int main(void) {
   S(LyapParams) S(LyapPoint) S(LyapCam) S(LyapRGBA) S(LyapView) S(LyapRequest)
   S(LyapVolumeMode) S(LyapRenderMode) S(LyapTuning) S(LyapServerConfig) S(LyapResult) S(LyapServerStats)
-  S(LyapEditor)
+  S(LyapEditor) S(LyapVolumeBlock)
+  O(LyapTuning, level_budget) O(LyapTuning, voxel_tile) O(LyapVolumeBlock, seq) O(LyapVolumeBlock, n)
+  O(LyapVolumeBlock, out)
   O(LyapCam, Q) O(LyapCam, ambient) O(LyapCam, chaosColor) O(LyapView, prm) O(LyapView, seq)
   O(LyapView, rgba) O(LyapView, num_lights) O(LyapRequest, view) O(LyapRequest, vol_prm)
   O(LyapRequest, vol_out) O(LyapRequest, deadline_ms) O(LyapServerConfig, mode) O(LyapServerConfig, vol_mode)
@@ -86,7 +88,7 @@ def test_struct_layouts_match_c(tmp_path):
           "LyapRGBA": lyap.RGBA, "LyapView": lyap.View, "LyapRequest": lyap.Request,
           "LyapVolumeMode": lyap.VolumeMode, "LyapRenderMode": lyap.RenderMode, "LyapTuning": lyap.Tuning,
           "LyapServerConfig": lyap.ServerConfig, "LyapResult": lyap.Result,
-          "LyapServerStats": lyap.ServerStats, "LyapEditor": lyap.Editor}
+          "LyapServerStats": lyap.ServerStats, "LyapEditor": lyap.Editor, "LyapVolumeBlock": lyap.VolumeBlock}
     for name, cls in py.items():
         assert C.sizeof(cls) == c[name][0], name
     assert c["LyapParams"][0] == 56 and c["LyapPoint"][0] == 36 and c["LyapCam"][0] == 224
@@ -101,7 +103,10 @@ def test_struct_layouts_match_c(tmp_path):
               "LyapServerStats.resident_bytes": lyap.ServerStats.resident_bytes,
               "LyapServerStats.device_ms": lyap.ServerStats.device_ms, "LyapEditor.cam": lyap.Editor.cam,
               "LyapEditor.lights": lyap.Editor.lights, "LyapEditor.home_light": lyap.Editor.home_light,
-              "LyapEditor.num_lights": lyap.Editor.num_lights, "LyapEditor.pending": lyap.Editor.pending}
+              "LyapEditor.num_lights": lyap.Editor.num_lights, "LyapEditor.pending": lyap.Editor.pending,
+              "LyapTuning.level_budget": lyap.Tuning.level_budget, "LyapTuning.voxel_tile": lyap.Tuning.voxel_tile,
+              "LyapVolumeBlock.seq": lyap.VolumeBlock.seq, "LyapVolumeBlock.n": lyap.VolumeBlock.n,
+              "LyapVolumeBlock.out": lyap.VolumeBlock.out}
     for k, f in fields.items():
         assert f.offset == c[k][0], k
 

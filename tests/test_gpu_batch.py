@@ -141,11 +141,14 @@ def test_autotune_keeps_results(ctx):
     ctx.sync()
     before = [b.fetch() for b in bufs]
     best = ctx.autotune(api, mode)
-    assert best.lanes_per_ray in (2, 4, 8, 16) and best.tile_rays > 0
+    # the built-in shape (zeros) or one of the candidates, one lane per ray included
+    assert best.lanes_per_ray in (0, 1, 2, 4, 8, 16) and best.slots in (0, 1, 2, 4)
+    assert all(b in (0, 16, 24, 32, 48, 64, 96, 128, 192, 256) for b in best.level_budget)
     ctx.render_batch(api, mode)
     ctx.sync()
     for b, e in zip(bufs, before):
         same(b.fetch(), e)
+    ctx.set_tuning(None)
 
 
 @pytest.mark.parametrize("margin", [0.0, 0.01])

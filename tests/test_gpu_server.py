@@ -1,9 +1,11 @@
 """The serving loop (lyap_server_*) on the GPU.
 
  - views and volumes served together, with launches small enough that every
-   view is cut into pieces spread over several launches and both render
-   slots, give exactly the bytes of rendering or computing each request alone
-   (a ray's or a voxel's arithmetic never depends on what shares its launch);
+   view is cut into pieces spread over several launches and slots, give
+   exactly the bytes of rendering or computing each request alone (a ray's
+   or a voxel's arithmetic never depends on what shares its launch), both
+   when launches carry views and volume slabs together and when volumes run
+   on slots of their own;
  - resident outputs (no caller buffers) hold the same bytes as caller buffers;
  - cancellation, the HBM budget's admission rule and the error paths.
 """
@@ -55,15 +57,19 @@ def volume_args(spec):
     return n, lo, hi, seq, prm
 
 
-@pytest.mark.parametrize("resident", [False, True])
-def test_mix_equals_alone(ctx, resident):
+@pytest.mark.parametrize("resident,flags,vrule", [(False, 0, "margin"), (True, 0, "strict"),
+                                                 (False, lyap.SERVER_SEPARATE_VOLUMES, "margin"),
+                                                 (True, lyap.SERVER_SEPARATE_VOLUMES, "strict")])
+def test_mix_equals_alone(ctx, resident, flags, vrule):
+    """Launches that carry views and volume slabs together (the default) and
+    separate view and volume slots give the bytes of each request alone."""
     torch = torch_cuda()
     views, lights, n_lights = served_views()
     ld = device_lights(lights, n_lights)
     mode = W.c2_mode(fp64_margin=0.0)
-    vmode = lyap.volume_mode(lyap.PREC_PRODUCT, fp64_margin=0.01)
+    vmode = lyap.volume_mode(**(W.C3_MODE if vrule == "margin" else W.C3_STRICT))
     cfg = lyap.server_config(render_slots=2, volume_slots=1, slice_ms=0.02, chunk_rays=128,
-                             mode=mode, vol_mode=vmode)
+                             mode=mode, vol_mode=vmode, flags=flags)
     srv = lyap.Server(ctx, cfg)
     bufs, vbufs, tickets = [], [], []
     # interleave views and volumes in submission order
