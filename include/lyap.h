@@ -263,8 +263,9 @@ int lyap_shade_points(lyap_ctx *ctx, LyapRGBA *d_rgba, const LyapPoint *d_points
  * {rounds, rays started, start clock, end clock, clocks evaluating, clocks
  * refilling, clocks combining, clocks of those refilling spent claiming
  * tiles, clocks of those combining spent handing rays over, clocks of
- * those refilling spent starting rays, 0 x 6}; NULL switches it off. No
- * reference counterpart. */
+ * those refilling spent starting rays, start and end on the constant-rate
+ * wall clock (100 MHz, common to the chip's XCDs, which the shader clock
+ * is not), 0 x 4}; NULL switches it off. No reference counterpart. */
 int lyap_debug_wave_log(lyap_ctx *ctx, void *d_log);
 
 /* Kernel timing: while on, every render kernel launch on the context's

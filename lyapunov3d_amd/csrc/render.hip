@@ -1109,6 +1109,8 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
     }
     uint32_t K = K0;                  /* lanes per ray; wave-uniform */
     const unsigned long long t_start = a.wave_log ? clock64() : 0ull;
+    /* the constant-rate clock, common to every XCD, for the launch's timeline */
+    const unsigned long long w_start = a.wave_log ? wall_clock64() : 0ull;
     unsigned long long c_eval = 0, c_refill = 0, c_combine = 0, c_claim = 0, c_hand = 0;
     unsigned long long c_start = 0;
     unsigned long long c_mark = t_start;
@@ -1484,7 +1486,9 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
         o[7] = c_claim;
         o[8] = c_hand;
         o[9] = c_start;
-        for (uint32_t k = 10; k < kWaveLogWords; ++k) o[k] = 0;
+        o[10] = w_start;
+        o[11] = wall_clock64();
+        for (uint32_t k = 12; k < kWaveLogWords; ++k) o[k] = 0;
     }
 }
 

@@ -66,7 +66,9 @@ def test_wave_log_accounts_for_every_ray(ctx, precision):
         assert (ran[:, 7] <= ran[:, 5]).all()               # tile claims are refill time
         assert (ran[:, 9] <= ran[:, 5]).all()               # ray starts are refill time
         assert (ran[:, 8] <= ran[:, 6]).all()               # hand-overs are combine time
-        assert (ran[:, 10:] == 0).all()
+        # the constant-rate clock brackets the wave too
+        assert (ran[:, 11] >= ran[:, 10]).all() and (ran[:, 10] > 0).all()
+        assert (ran[:, 12:] == 0).all()
         # a wave that started rays evaluated them
         assert (ran[ran[:, 1] > 0, 0] > 0).all()
 
