@@ -56,8 +56,23 @@ constexpr float kLn2f = 0.693147180559945309f;
 constexpr double kLn2 = 0.693147180559945309417232121458;
 /* A product block ending below the smallest normal float may have lost bits
  * to gradual underflow; one ending at or above it did not, since |u| <= 1
- * keeps the product non-increasing within the block. */
+ * keeps the product non-increasing within the block. Such a block is
+ * recomputed with a renormalisation per step. */
 constexpr float kUnderflowGuard = 0x1p-126f;
+
+/* The per-period volume kernels redo a block that ended at infinity (or
+ * NaN) as well: the orbit of a point with r > 4 (a volume reaching past 4)
+ * can escape [0, 1], and then |u| > 1 grows fast and may overflow between
+ * renormalisations where its factors are finite. Redone per step the product
+ * stays finite while its factors are, so their adaptive block lengths never
+ * change a bit, and lambda is finite exactly when every factor is (as with
+ * the reference's sum of logs). The window forms of the persistent launch
+ * keep the one compare: their samples lie in the [lMin, lMax] box (at most
+ * an ulp past it), and this check costs the render kernels registers. */
+__device__ __forceinline__ bool redo_block(float M)
+{
+    return !(M >= kUnderflowGuard && M <= 0x1.fffffep127f);
+}
 
 /* Wave-uniform description of one lambda evaluation: iteration counts,
  * sequence, and host-derived loop bounds. Built by make_eval_desc(). */
@@ -75,9 +90,10 @@ struct EvalDesc {
     /* first part of the window for the convergence check: the head plus
      * half_periods full periods (half_steps steps) */
     uint32_t half_periods, half_steps;
-    /* product form: full periods per renormalisation block */
+    /* product form: full periods per renormalisation block, and after a
+     * block that had to be redone */
     uint32_t block_periods;
-    uint32_t pad;
+    uint32_t block_short;
     /* occurrences of letters A..D in the whole window and in its first part */
     float count[4];
     float count_half[4];
@@ -98,6 +114,15 @@ struct EvalDesc {
 };
 
 inline void rebuild_blocks(EvalDesc *e, uint32_t W);
+
+/* Default product blocks of the per-period volume kernels: whole periods
+ * making up to kBlockLong steps, and after the first block that had to be
+ * redone (see redo_block) the whole periods nearest kBlockShort. The whole wave waits for any
+ * lane's redo: where lambda is strongly negative (the stable regions of the
+ * [2,4]^3 volumes) 30-step blocks redo about half the time and 10-step ones
+ * rarely, while in chaotic regions long blocks redo as rarely and
+ * renormalise less often. */
+constexpr uint32_t kBlockLong = 32, kBlockShort = 12;
 
 /* Host: derive the loop bounds. Returns false for an unusable sequence. */
 inline bool make_eval_desc(EvalDesc *e, const LyapParams &prm,
@@ -130,9 +155,11 @@ inline bool make_eval_desc(EvalDesc *e, const LyapParams &prm,
     e->tail = left % L;
     e->half_periods = e->acc_periods / 2;
     e->half_steps = (e->head_end - e->head_begin) + e->half_periods * (uint32_t)L;
-    if (renorm == 0) renorm = 32;
-    uint32_t bp = renorm / (uint32_t)L;
+    uint32_t bp = (renorm ? renorm : kBlockLong) / (uint32_t)L;
     e->block_periods = bp ? bp : 1;
+    /* an explicit length stays; the default shortens after a redo */
+    bp = renorm ? e->block_periods : (kBlockShort + (uint32_t)L / 2u) / (uint32_t)L;
+    e->block_short = bp ? bp : 1;
     /* letter counts */
     for (int k = 0; k < 4; ++k) { e->count[k] = 0.f; e->count_half[k] = 0.f; }
     for (uint32_t j = e->head_begin; j < e->head_end; ++j) {
@@ -153,7 +180,10 @@ inline void rebuild_blocks(EvalDesc *e, uint32_t W)
 {
     const uint32_t L = (uint32_t)e->L;
     e->W = W;
-    e->B = L * (W / L);
+    /* a 32-slot window with a period of at most 16 takes two 16-slot blocks
+     * at a time, so the FP64 form, renormalised at half and end of each
+     * block, renormalises after the same steps in either window */
+    e->B = (W > 16 && L <= 16) ? 2u * (L * (16u / L)) : L * (W / L);
     for (uint32_t k = 0; k < 4; ++k) e->wmask[k] = 0;
     for (uint32_t j = 0; j < (uint32_t)LYAP_MAX_SEQ; ++j) {
         e->wletters[j] = e->letters[j % L];
@@ -293,14 +323,16 @@ __device__ __forceinline__ float umap(float u, float b)
 }
 
 /* `n` full periods in blocks of `bp` periods, renormalising after each block
- * and recomputing a block whose product fell below kUnderflowGuard. */
+ * and recomputing a block whose product redo_block() flags; once a lane of
+ * the wave has needed that, the blocks are `bshort` periods long. */
 template <int L>
 __device__ __forceinline__ void product_periods(float &u, ProductState &st,
                                                 const float (&a)[L], const float (&b)[L],
-                                                uint32_t n, uint32_t bp)
+                                                uint32_t n, uint32_t &bp, uint32_t bshort)
 {
-    for (uint32_t p = 0; p < n; p += bp) {
+    for (uint32_t p = 0; p < n;) {
         const uint32_t cnt = n - p < bp ? n - p : bp;
+        p += cnt;
         const float u0 = u;
         const ProductState s0 = st;
         for (uint32_t k = 0; k < cnt; ++k) {
@@ -310,7 +342,8 @@ __device__ __forceinline__ void product_periods(float &u, ProductState &st,
                 st.M *= __builtin_fabsf(u);
             }
         }
-        if (st.M < kUnderflowGuard) {
+        const bool rd = redo_block(st.M);
+        if (rd) {
             float w = u0;
             st = s0;
             for (uint32_t k = 0; k < cnt; ++k) {
@@ -322,6 +355,7 @@ __device__ __forceinline__ void product_periods(float &u, ProductState &st,
                 }
             }
         }
+        if (__any(rd)) bp = bshort;
         renorm(st);
     }
 }
@@ -365,12 +399,13 @@ __device__ __forceinline__ ProductResult lambda_product(float x, float y,
         }
     }
     ProductState st_half{1.0f, 0};
+    uint32_t bp = e.block_periods;
     if (HALVES) {
-        product_periods<L>(u, st, a, b, e.half_periods, e.block_periods);
+        product_periods<L>(u, st, a, b, e.half_periods, bp, e.block_short);
         st_half = st;
-        product_periods<L>(u, st, a, b, e.acc_periods - e.half_periods, e.block_periods);
+        product_periods<L>(u, st, a, b, e.acc_periods - e.half_periods, bp, e.block_short);
     } else {
-        product_periods<L>(u, st, a, b, e.acc_periods, e.block_periods);
+        product_periods<L>(u, st, a, b, e.acc_periods, bp, e.block_short);
     }
 #pragma unroll
     for (int j = 0; j < L; ++j) {
@@ -919,12 +954,18 @@ __device__ __forceinline__ float lambda_plain_w(float x, float y, float z,
 
 /* FP64 with the reference's map ((r*v)*(1-v), separately rounded, so the
  * orbit is the IEEE double one); the derivative r(1 - 2v) only feeds the
- * log sum and is formed with one fma. */
+ * log sum and is formed with one fma. The product is renormalised at the end
+ * of each block and, in a 32-slot window, after its first `mid` = B/2
+ * positions: at most 16 factors between renormalisations, each at most 4 in
+ * magnitude while the orbit stays in [0, 1], and the same steps in either
+ * window (B is then twice the 16-slot block). An orbit that escapes (a
+ * sample just outside the box) can overflow between renormalisations; that
+ * happens after the same steps wherever the window or launch is. */
 template <int W, bool FULL, bool ACC>
 __device__ __forceinline__ void fp64_block(double &v, ProductState64 &st,
                                            const double (&r)[W],
                                            const double (&m2r)[W], uint32_t lo,
-                                           uint32_t hi)
+                                           uint32_t hi, uint32_t mid)
 {
 #pragma unroll
     for (int j = 0; j < W; ++j) {
@@ -932,7 +973,7 @@ __device__ __forceinline__ void fp64_block(double &v, ProductState64 &st,
             v = (r[j] * v) * (1.0 - v);
             if (ACC) {
                 st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
-                if (W > 16 && j == 15) renorm64(st);
+                if (W > 16 && (uint32_t)j + 1u == mid) renorm64(st);
             }
         }
     }
@@ -956,7 +997,7 @@ struct Fp64Full {
                 v = (r[j] * v) * (1.0 - v);
                 if (ACC) {
                     st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
-                    if (W > 16 && j == 15) renorm64(st);
+                    if (W > 16 && j + 1 == B / 2) renorm64(st);
                 }
             }
             if (ACC) renorm64(st);
@@ -965,16 +1006,17 @@ struct Fp64Full {
 };
 
 /* FP64 steps at window positions [LO, HI), guarded past W/2 as in
- * product_range; the product is renormalised after position 15 (W = 32)
- * and at the end: exact power-of-two scalings, and the final (E, M) with M
- * in [0.5, 1) is the same wherever the earlier ones fell. */
+ * product_range; the product is renormalised after position mid - 1 (a
+ * 32-slot window) and at the end, as in fp64_block: exact power-of-two
+ * scalings, and the final (E, M) with M in [0.5, 1) is the same wherever
+ * the earlier ones fell. */
 template <int W, bool ACC>
 struct Fp64Range {
     double &v;
     ProductState64 &st;
     const double (&r)[W];
     const double (&m2r)[W];
-    uint32_t hi;
+    uint32_t hi, mid;
     template <int LO, int HI, bool GUARD>
     __device__ __forceinline__ void run()
     {
@@ -984,7 +1026,7 @@ struct Fp64Range {
                 v = (r[j] * v) * (1.0 - v);
                 if (ACC) {
                     st.M *= __builtin_fabs(__builtin_fma(m2r[j], v, r[j]));
-                    if (W > 16 && j == 15) renorm64(st);
+                    if (W > 16 && (uint32_t)j + 1u == mid) renorm64(st);
                 }
             }
         }
@@ -1022,28 +1064,29 @@ __device__ __forceinline__ double lambda_fp64_w(float xf, float yf, float zf,
     }
     double v = 0.5;
     ProductState64 st{1.0, 0};
+    const uint32_t mid = e.B / 2u;
     {
         Fp64Full<W, false> f{v, st, r, m2r, e.set_blocks};
         with_block_len<W, W>(e.B, f);
     }
     {
-        Fp64Prefix<W, false> f{{v, st, r, m2r, 0}};
+        Fp64Prefix<W, false> f{{v, st, r, m2r, 0, mid}};
         with_len<W, W - 1>(e.set_rem, f);
     }
     const double off = v - 0.5;
     if (!(off <= -1e-8 || off >= 1e-8)) return 0.0 / (double)e.accum;
     if (e.head_hi == e.B) {
-        Fp64Head<W, true> f{{v, st, r, m2r, e.B}};
+        Fp64Head<W, true> f{{v, st, r, m2r, e.B, mid}};
         with_len<W, W - 1>(e.head_lo, f);
     } else {
-        fp64_block<W, false, true>(v, st, r, m2r, e.head_lo, e.head_hi);
+        fp64_block<W, false, true>(v, st, r, m2r, e.head_lo, e.head_hi, mid);
     }
     {
         Fp64Full<W, true> f{v, st, r, m2r, e.acc_blocks};
         with_block_len<W, W>(e.B, f);
     }
     {
-        Fp64Prefix<W, true> f{{v, st, r, m2r, 0}};
+        Fp64Prefix<W, true> f{{v, st, r, m2r, 0, mid}};
         with_len<W, W - 1>(e.acc_rem, f);
     }
     const double lam = ((double)st.E * kLn2 + log(st.M)) / (double)e.accum;
