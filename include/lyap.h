@@ -144,9 +144,9 @@ enum {
 /* Volume options. NULL selects {LYAP_PREC_PLAIN, no re-evaluation}. */
 typedef struct LyapVolumeMode {
     int32_t precision;      /* LYAP_PREC_* */
-    uint32_t renorm;        /* PRODUCT: steps between renormalisations, 0 = the whole
-                               periods nearest 12 steps (the results do not depend
-                               on it) */
+    uint32_t renorm;        /* PRODUCT: steps between renormalisations; 0 = whole
+                               periods up to 32 steps, about 12 once a wave has
+                               had to redo a block (results do not depend on it) */
     float fp64_margin;      /* re-evaluate in FP64 where |lambda| < margin; 0 = off */
     float converge_tau;     /* also re-evaluate where the two halves of the window */
     float converge_band;    /*   differ by > tau and the smaller half-lambda < band */
