@@ -220,6 +220,27 @@ def test_edge_cases(ctx, prec):
     assert np.all(np.isnan(g))
 
 
+@pytest.mark.parametrize("rule", ["product", "strict"])
+def test_product_block_length_changes_no_bit(ctx, rule):
+    """The product form's renormalisation is exact and a block whose product
+    underflowed (or overflowed) is redone step by step, so the block length
+    never changes a bit: the default (whole periods up to 32 steps, shorter
+    once a wave has redone a block) gives the bytes of every fixed length.
+    On C1, where strongly negative lambda makes long blocks underflow, and on
+    a box reaching past 4, where orbits escape [0, 1] and the product grows;
+    with the window halves of the convergence rule as well."""
+    base = dict(W.C3_STRICT) if rule == "strict" else dict(precision=lyap.PREC_PRODUCT)
+    prm, seq, lo, hi, n = W.volume_config("c1")
+    cases = [(prm, seq, lo, hi, n),
+             (lyap.make_params(settle=30, accum=300), "ABBAC", (3.9, 3.9, 3.9), (4.05, 4.05, 4.05), 24)]
+    for prm, seq, lo, hi, n in cases:
+        ref = gpu_volume(ctx, prm, seq, lo, hi, n, lyap.volume_mode(**base)).view(np.uint32)
+        for renorm in (5, 10, 15, 40, 64):
+            m = lyap.volume_mode(**dict(base, renorm=renorm))
+            got = gpu_volume(ctx, prm, seq, lo, hi, n, m).view(np.uint32)
+            assert np.array_equal(got, ref), (lo, renorm)
+
+
 def test_errors(ctx):
     out = empty(64)
     prm = lyap.make_params()
