@@ -493,6 +493,9 @@ typedef struct LyapServerConfig {
                                           launch may carry both kinds */
     LyapRenderMode mode;
     LyapVolumeMode vol_mode;
+    uint32_t max_queued;               /* requests waiting for admission; a submit
+                                          beyond it returns LYAP_ERR_QUEUE_FULL
+                                          (0: no cap) */
 } LyapServerConfig;
 
 typedef struct LyapResult {
@@ -524,13 +527,22 @@ typedef struct LyapServerStats {
  * lyap_run_batch), each slot on its own stream, so the launches' work
  * overlaps on the device. A request's outputs are admitted against
  * `hbm_budget_bytes` when its first piece is issued and stay resident until
- * lyap_server_release. One host thread drives a server. */
+ * lyap_server_release. Requests are admitted in deadline order: one that
+ * does not fit yet holds back the admission of those with later deadlines
+ * (requests already admitted keep running), so a large request is not
+ * passed over indefinitely. A request without a deadline orders after every
+ * request that has one. When the device has no room for a request's outputs,
+ * the server first frees its pooled buffers; the request then waits while
+ * other requests hold resident outputs, and ends with LYAP_ERR_DEVICE when
+ * none do. A launch that fails ends the requests it carried with its error
+ * once their other pieces are done. One host thread drives a server. */
 typedef struct lyap_server lyap_server;
 int lyap_server_create(lyap_ctx *ctx, const LyapServerConfig *cfg,
                        lyap_server **out);
 void lyap_server_destroy(lyap_server *srv);
 /* Queues a request. A VIEW's lights and any caller-given outputs must stay
- * valid until the request finishes. */
+ * valid until the request finishes. LYAP_ERR_QUEUE_FULL when `max_queued`
+ * requests already wait for admission. */
 int lyap_server_submit(lyap_server *srv, const LyapRequest *req,
                        uint64_t *ticket);
 /* Advances the server: retires finished launches, admits requests, and

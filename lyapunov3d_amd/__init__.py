@@ -177,6 +177,7 @@ class ServerConfig(C.Structure):
         ("hbm_budget_bytes", C.c_size_t), ("render_slots", C.c_uint32),
         ("volume_slots", C.c_uint32), ("slice_ms", C.c_double), ("chunk_rays", C.c_uint32),
         ("flags", C.c_uint32), ("mode", RenderMode), ("vol_mode", VolumeMode),
+        ("max_queued", C.c_uint32),
     ]
 
 
@@ -701,9 +702,11 @@ SERVER_SEPARATE_VOLUMES = 1
 
 def server_config(hbm_budget_bytes: int = 0, render_slots: int = 0, volume_slots: int = 0,
                   slice_ms: float = 0.0, chunk_rays: int = 0, mode: Optional[RenderMode] = None,
-                  vol_mode: Optional[VolumeMode] = None, flags: int = 0) -> ServerConfig:
+                  vol_mode: Optional[VolumeMode] = None, flags: int = 0,
+                  max_queued: int = 0) -> ServerConfig:
     c = ServerConfig()
     c.flags = flags
+    c.max_queued = max_queued
     c.hbm_budget_bytes = hbm_budget_bytes
     c.render_slots, c.volume_slots = render_slots, volume_slots
     c.slice_ms, c.chunk_rays = slice_ms, chunk_rays

@@ -49,6 +49,7 @@
 #include <stdint.h>
 
 #include "lyap.h"
+#include "sequence.hpp"
 
 namespace lyap {
 
@@ -128,12 +129,8 @@ constexpr uint32_t kBlockLong = 32, kBlockShort = 12;
 inline bool make_eval_desc(EvalDesc *e, const LyapParams &prm,
                            const int32_t *seq, uint32_t renorm)
 {
-    int L = 0;
-    while (L <= LYAP_MAX_SEQ && seq[L] != -1) {
-        if (seq[L] < 0 || seq[L] > 3) return false;
-        ++L;
-    }
-    if (L == 0 || L > LYAP_MAX_SEQ) return false;
+    const int L = sequence_period(seq);
+    if (L == 0) return false;
     *e = EvalDesc{};
     e->settle = prm.settle;
     e->accum = prm.accum;

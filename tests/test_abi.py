@@ -64,6 +64,7 @@ int main(void) {
   O(LyapView, rgba) O(LyapView, num_lights) O(LyapRequest, view) O(LyapRequest, vol_prm)
   O(LyapRequest, vol_out) O(LyapRequest, deadline_ms) O(LyapServerConfig, mode) O(LyapServerConfig, vol_mode)
   O(LyapResult, submit_ms) O(LyapResult, lambda) O(LyapServerConfig, slice_ms)
+  O(LyapServerConfig, max_queued)
   O(LyapServerStats, resident_bytes) O(LyapServerStats, device_ms)
   O(LyapEditor, cam) O(LyapEditor, lights) O(LyapEditor, home_light) O(LyapEditor, num_lights)
   O(LyapEditor, pending)
@@ -100,6 +101,7 @@ def test_struct_layouts_match_c(tmp_path):
               "LyapRequest.deadline_ms": lyap.Request.deadline_ms, "LyapServerConfig.mode": lyap.ServerConfig.mode,
               "LyapServerConfig.vol_mode": lyap.ServerConfig.vol_mode, "LyapResult.submit_ms": lyap.Result.submit_ms,
               "LyapResult.lambda": lyap.Result.lambda_, "LyapServerConfig.slice_ms": lyap.ServerConfig.slice_ms,
+              "LyapServerConfig.max_queued": lyap.ServerConfig.max_queued,
               "LyapServerStats.resident_bytes": lyap.ServerStats.resident_bytes,
               "LyapServerStats.device_ms": lyap.ServerStats.device_ms, "LyapEditor.cam": lyap.Editor.cam,
               "LyapEditor.lights": lyap.Editor.lights, "LyapEditor.home_light": lyap.Editor.home_light,
