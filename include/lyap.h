@@ -224,12 +224,17 @@ typedef struct LyapRenderMode {
 /* How rays are spread over lanes. BATCHED: the persistent work queue with
  * lane groups per ray, refill of finished lanes and continuation launches;
  * with bisection, rays whose march found its opaque sample are refined in a
- * launch of their own, two lanes per ray.
+ * launch of their own, two lanes per ray. A small launch of views refined by
+ * bisection (fewer rays than about two per resident lane) marches in passes
+ * instead: a set-up launch lists the rays that meet the box, then launches
+ * of one round each give every listed ray K lanes (4, 8, 16, 32, then 64
+ * samples at once), the last running its rays to the end of their march.
+ * LEVELS: BATCHED without the passes (the persistent launches only).
  * DIRECT: the reference's own launch shape (kernel.cu:500-516): one lane
  * per ray for 64 consecutive rays (an 8x8 pixel block), each lane marching
  * its ray to the end and a wave taking new rays only when all 64 are done.
- * Both give the same bytes; DIRECT is the plain per-request baseline. */
-enum { LYAP_SCHEDULE_BATCHED = 0, LYAP_SCHEDULE_DIRECT = 1 };
+ * All give the same bytes; DIRECT is the plain per-request baseline. */
+enum { LYAP_SCHEDULE_BATCHED = 0, LYAP_SCHEDULE_DIRECT = 1, LYAP_SCHEDULE_LEVELS = 2 };
 
 /* kernel_calc_render (kernel.hpp:17, kernel.cu:500-516; launched at
  * lyap_interactive.cu:711): one primary ray per pixel of a w*h image through
@@ -260,7 +265,8 @@ int lyap_shade_points(lyap_ctx *ctx, LyapRGBA *d_rgba, const LyapPoint *d_points
                       unsigned num_lights, unsigned w, unsigned h);
 
 /* Diagnostic: a device buffer of 16 uint64 per wave slot, 8192 slots per
- * launch level, 4 levels (4 MiB), that later render launches fill with
+ * launch level, 4 levels (4 MiB), that later render launches (in the
+ * persistent levels, which frames use while the buffer is set) fill with
  * {rounds, rays started, start clock, end clock, clocks evaluating, clocks
  * refilling, clocks combining, clocks of those refilling spent claiming
  * tiles, clocks of those combining spent handing rays over, clocks of

@@ -50,6 +50,7 @@ REQ_VOLUME = 1
 
 SCHEDULE_BATCHED = 0
 SCHEDULE_DIRECT = 1
+SCHEDULE_LEVELS = 2
 
 (EDIT_LOCK, EDIT_ROTPAN, EDIT_MAG, EDIT_NTHRESH, EDIT_OTHRESH, EDIT_CTHRESH, EDIT_D, EDIT_JITTER,
  EDIT_L_PAN, EDIT_L_RANGE, EDIT_L_COLOR) = range(11)

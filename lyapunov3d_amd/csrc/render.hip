@@ -23,6 +23,7 @@
  */
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -57,6 +58,7 @@ struct ViewDesc {
     uint32_t cont_cap;
     uint32_t refine_base;      /* this view's region of the refinement records */
     uint32_t refine_cap;       /* records per sub-region (kRefineSubs of them) */
+    uint32_t ray_base;         /* march passes: first ray of this view in ray_start's grid */
     /* Items of a launch are views or volume blocks (kItemVolume). A volume
      * block is z slices [vz0, vz0 + n_rays / (vnx * vny)) of an vnx * vny * vnz
      * grid over [vlo, vlo + vspan), lambda written to vol_out (slice vz0) in
@@ -102,6 +104,12 @@ static_assert(sizeof(RefineRec) == 48, "refinement record size");
  * that waves of several XCDs update are slow. */
 constexpr uint32_t kRefineSubs = 8;
 
+/* The march passes list their rays the same way: a view's continuation
+ * region is kContSubs sub-lists of cont_cap / kContSubs records, one counter
+ * per sub-list on its own 128-byte line, taken by block index modulo
+ * kContSubs. */
+constexpr uint32_t kContSubs = 8;
+
 struct RenderArgs {
     const ViewDesc *views;
     uint32_t n_views;
@@ -122,7 +130,9 @@ struct RenderArgs {
     uint32_t resident_lanes;
     uint32_t direct;           /* one lane per ray, no refill until the wave is idle */
     const ContEntry *cont_in;
-    const uint32_t *in_counts;         /* per view */
+    const uint32_t *in_counts;         /* per view, or per view and sub-list (in_subs) */
+    uint32_t in_subs;                  /* 0: one input list per view; kContSubs: the
+                                          march passes' per-XCD sub-lists */
     ContEntry *cont_out;
     uint32_t *out_counts;              /* per view */
     /* bisection rays park here when their march ends (0: refine in place);
@@ -746,6 +756,80 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* The march part of combine: the group's samples j = 0..K-1 (read from the
+ * stash at base + j, N entries) were taken at the positions the sequential
+ * method would reach if none of them stopped it. Find the first that does,
+ * account the transparent ones before it in order, then consume it. Sets
+ * `finished` when this lane's ray ended (a miss, written by the leader). */
+template <class Stash, uint32_t N>
+__device__ __forceinline__ void combine_march(const ViewDesc &v, Ray &r, uint64_t evm, const Stash &st,
+                                              uint32_t base, uint32_t K, int ph, bool leader,
+                                              bool &finished)
+{
+    const bool marching = ph == PH_MARCH;
+    const uint32_t e = evm ? (uint32_t)__ffsll((unsigned long long)evm) - 1u : K;
+    /* transparent samples 0..e-1 added in order (kernel.cu:370-375). Each
+     * sample's share was formed when it was evaluated; a sample that adds
+     * to neither sum holds +0, and adding +0 leaves a sum unchanged (the
+     * sums start at +0 and so are never -0), so the bits are those of the
+     * sequential method. Four loads are issued ahead of their adds. */
+    for (uint32_t j0 = 0; __ballot(marching && j0 < e) != 0; j0 += 4u) {
+        float c4[4], a4[4];
+#pragma unroll
+        for (uint32_t q = 0; q < 4u; ++q) {
+            const uint32_t i = min(base + j0 + q, N - 1u);
+            c4[q] = st.ca[i];
+            a4[q] = st.aa[i];
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < 4u; ++q) {
+            const bool take = marching && j0 + q < e;
+            r.c = take ? r.c + c4[q] : r.c;
+            r.a = take ? r.a + a4[q] : r.a;
+        }
+    }
+    if (marching && e > 0) {
+        r.evals += e;
+        r.launch_samples += e;
+        r.index += e;
+        r.l = st.lam[min(base + e - 1u, N - 1u)];
+    }
+    /* position and lambda of sample e */
+    const uint32_t src = min(base + (e < K ? e : K - 1u), N - 1u);
+    const float te = st.t[src];
+    const F3 Pe = {st.x[src], st.y[src], st.z[src]};
+    const float le = st.lam[src];
+    const uint32_t ve = st.ok[src];
+    if (marching) {
+        if (e > 0) {
+            r.last_step = r.dt;
+            if (v.refine == LYAP_REFINE_BISECT) r.m += (e < K ? e : K - 1u);
+        }
+        if (e < K) {
+            r.t = te;
+            r.P = Pe;
+            if (!ve) {
+                /* the step to sample e left the box or could not advance */
+                if (leader) write_miss(v, r);
+                r.phase = PH_IDLE;
+                finished = true;
+            } else {
+                march_consume(v, r, le, leader);
+                if (r.phase == PH_IDLE) finished = true;
+            }
+        } else {
+            /* all K transparent: step from sample K-1 to sample K */
+            r.t = te;
+            r.P = Pe;
+            if (r.index >= v.max_samples || !march_step(v, r, r.dt) || r.t > r.t1) {
+                if (leader) write_miss(v, r);
+                r.phase = PH_IDLE;
+                finished = true;
+            }
+        }
+    }
+}
+
 /* Combines the group's K samples (read from the round's stash at
  * base + j) and advances the replicated ray state in the sequential
  * method's order. A group's lanes share one phase, and `evm` holds one bit
@@ -761,74 +845,8 @@ __device__ bool combine(const ViewDesc &v, Ray &r, uint64_t evm, const RoundStas
     const bool bisect = v.refine == LYAP_REFINE_BISECT;
     bool finished = false;
 
-    /* march: the group's samples j = 0..K-1 were taken at the positions the
-     * sequential method would reach if none of them stopped it. Find the
-     * first that does, account the transparent ones before it in order, then
-     * consume it. */
-    if (__ballot(ph == PH_MARCH) != 0) {
-        const bool marching = ph == PH_MARCH;
-        const uint32_t e = evm ? (uint32_t)__ffsll((unsigned long long)evm) - 1u : K;
-        /* transparent samples 0..e-1 added in order (kernel.cu:370-375). Each
-         * sample's share was formed when it was evaluated; a sample that adds
-         * to neither sum holds +0, and adding +0 leaves a sum unchanged (the
-         * sums start at +0 and so are never -0), so the bits are those of the
-         * sequential method. Four loads are issued ahead of their adds. */
-        for (uint32_t j0 = 0; __ballot(marching && j0 < e) != 0; j0 += 4u) {
-            float c4[4], a4[4];
-#pragma unroll
-            for (uint32_t q = 0; q < 4u; ++q) {
-                const uint32_t i = min(base + j0 + q, 64u * kMaxSlots - 1u);
-                c4[q] = st.ca[i];
-                a4[q] = st.aa[i];
-            }
-#pragma unroll
-            for (uint32_t q = 0; q < 4u; ++q) {
-                const bool take = marching && j0 + q < e;
-                r.c = take ? r.c + c4[q] : r.c;
-                r.a = take ? r.a + a4[q] : r.a;
-            }
-        }
-        if (marching && e > 0) {
-            r.evals += e;
-            r.launch_samples += e;
-            r.index += e;
-            r.l = st.lam[min(base + e - 1u, 64u * kMaxSlots - 1u)];
-        }
-        /* position and lambda of sample e */
-        const uint32_t src = min(base + (e < K ? e : K - 1u), 64u * kMaxSlots - 1u);
-        const float te = st.t[src];
-        const F3 Pe = {st.x[src], st.y[src], st.z[src]};
-        const float le = st.lam[src];
-        const uint32_t ve = st.ok[src];
-        if (marching) {
-            if (e > 0) {
-                r.last_step = r.dt;
-                if (v.refine == LYAP_REFINE_BISECT) r.m += (e < K ? e : K - 1u);
-            }
-            if (e < K) {
-                r.t = te;
-                r.P = Pe;
-                if (!ve) {
-                    /* the step to sample e left the box or could not advance */
-                    if (leader) write_miss(v, r);
-                    r.phase = PH_IDLE;
-                    finished = true;
-                } else {
-                    march_consume(v, r, le, leader);
-                    if (r.phase == PH_IDLE) finished = true;
-                }
-            } else {
-                /* all K transparent: step from sample K-1 to sample K */
-                r.t = te;
-                r.P = Pe;
-                if (r.index >= v.max_samples || !march_step(v, r, r.dt) || r.t > r.t1) {
-                    if (leader) write_miss(v, r);
-                    r.phase = PH_IDLE;
-                    finished = true;
-                }
-            }
-        }
-    }
+    if (__ballot(ph == PH_MARCH) != 0)
+        combine_march<RoundStash, 64u * kMaxSlots>(v, r, evm, st, base, K, ph, leader, finished);
 
     /* bisection: follow the actual path through this round's tree levels,
      * reading each node's side of the threshold from `evm`; the lambda of the
@@ -922,6 +940,14 @@ __device__ __forceinline__ uint32_t uniform(uint32_t x)
     return __builtin_amdgcn_readfirstlane(x);
 }
 
+/* Listed rays of view v's sub-list s (subs 0: the view's one list). */
+__device__ __forceinline__ uint32_t cont_count(const ViewDesc *__restrict__ views, const uint32_t *counts,
+                                               uint32_t subs, uint32_t v, uint32_t s)
+{
+    if (subs == 0) return uniform(min(counts[v], views[uniform(v)].cont_cap));
+    return uniform(min(counts[(v * kContSubs + s) * kHeadStride], views[uniform(v)].cont_cap / kContSubs));
+}
+
 /* Rays [first, last) of view `view` for tile t. Level 0 tiles cover the
  * views' pixel blocks; later levels cover each view's handed-over rays,
  * `tile` entries per tile. Views are few: scalar scans. Returns false when t
@@ -943,16 +969,21 @@ __device__ __forceinline__ bool tile_lookup(const ViewDesc *__restrict__ views,
         return true;
     }
     uint32_t acc = 0;
+    const uint32_t ns = a.in_subs ? kContSubs : 1u;
     for (uint32_t v = 0; v < a.n_views; ++v) {
-        const uint32_t cnt = uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
-        const uint32_t nt = (cnt + tile - 1) / tile;
-        if (t < acc + nt) {
-            view = uniform(v);
-            first = uniform((t - acc) * tile);
-            last = uniform(min(first + tile, cnt));
-            return true;
+        const uint32_t sub_cap = a.in_subs ? uniform(views[uniform(v)].cont_cap / kContSubs) : 0u;
+        for (uint32_t s = 0; s < ns; ++s) {
+            const uint32_t cnt = cont_count(views, a.in_counts, a.in_subs, v, s);
+            const uint32_t nt = (cnt + tile - 1) / tile;
+            if (t < acc + nt) {
+                /* positions within the view's region: sub-list s starts at s * sub_cap */
+                view = uniform(v);
+                first = uniform(s * sub_cap + (t - acc) * tile);
+                last = uniform(min(first + tile, s * sub_cap + cnt));
+                return true;
+            }
+            acc += nt;
         }
-        acc += nt;
     }
     return false;
 }
@@ -1086,7 +1117,8 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
          * ray and round as lanes would be if each took one */
         uint32_t total = 0;
         for (uint32_t v = 0; v < a.n_views; ++v)
-            total += uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+            for (uint32_t s = 0; s < (a.in_subs ? kContSubs : 1u); ++s)
+                total += cont_count(views, a.in_counts, a.in_subs, v, s);
         if (total == 0) return;
         K0 = 1;
         while (K0 < 64u && (uint64_t)a.resident_lanes >= (uint64_t)total * (2u * K0)) K0 *= 2;
@@ -1102,10 +1134,9 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
     uint32_t level_tiles = a.total_tiles;
     if (a.level > 0) {
         level_tiles = 0;
-        for (uint32_t v = 0; v < a.n_views; ++v) {
-            const uint32_t cnt = uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
-            level_tiles += (cnt + tile - 1u) / tile;
-        }
+        for (uint32_t v = 0; v < a.n_views; ++v)
+            for (uint32_t s = 0; s < (a.in_subs ? kContSubs : 1u); ++s)
+                level_tiles += (cont_count(views, a.in_counts, a.in_subs, v, s) + tile - 1u) / tile;
     }
     uint32_t K = K0;                  /* lanes per ray; wave-uniform */
     const unsigned long long t_start = a.wave_log ? clock64() : 0ull;
@@ -1492,6 +1523,218 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
     }
 }
 
+/* ---- march passes ------------------------------------------------------- */
+
+/* A small frame has fewer rays that meet the box than the chip has lanes, so
+ * its march speculates: K lanes of a ray evaluate the next K samples at once.
+ * In the persistent launch the waves also carry the refill, hand-over and
+ * shading state, which holds them to three per SIMD. March passes split the
+ * march instead into launches of one round each: march_pass gives each ray
+ * K lanes for one round (K samples, the persistent launch's sample
+ * positions, evaluation and combine) and lists the rays still marching for
+ * the next pass, K rising from pass to pass as the list shrinks; ray_start
+ * sets up the pixels' rays (one lane each) beforehand and lists those that
+ * meet the box, and the last pass runs its rays to the end of their march.
+ * Rays whose march ended park for the refinement launch as before. A ray's
+ * arithmetic is the same in every schedule, so the bytes are too. (Setting
+ * the rays up in the first pass instead, every lane of a group the same ray,
+ * raised the pass kernel's registers and made the frame 8 % slower.) */
+
+struct PassArgs {
+    uint32_t n_views;
+    uint32_t K;                        /* lanes (samples) per ray in this pass */
+    const ContEntry *cont_in;          /* march_pass: the rays to advance */
+    const uint32_t *in_counts;         /* per view and sub-list, kHeadStride words apart */
+    ContEntry *cont_out;               /* the rays still marching */
+    uint32_t *out_counts;
+    RefineRec *refine_out;
+    uint32_t *refine_counts;           /* per view and sub-region, kHeadStride words apart */
+    unsigned long long *eval_counter;
+    uint32_t rounds;                   /* rounds per ray (0: until the march ends) */
+};
+
+/* One wave's samples of a round (one per lane). */
+struct PassStash {
+    float lam[64], t[64], x[64], y[64], z[64];
+    uint32_t ok[64];
+    float ca[64], aa[64];
+};
+
+/* Appends the rays of this wave's lanes with `want` (leaders) to view vi's
+ * sub-list of this block's XCD (block index modulo kContSubs), one atomic
+ * per wave; rays that do not fit go on to the next sub-list. A view's
+ * sub-lists hold all its rays together, and a full sub-list has every one of
+ * its slots written, so each ray finds a slot within kContSubs tries. */
+__device__ __forceinline__ void append_cont(const ViewDesc &v, uint32_t vi, const Ray &r, bool want,
+                                            ContEntry *out, uint32_t *counts)
+{
+    const uint32_t lane = __lane_id();
+    const uint32_t sub_cap = v.cont_cap / kContSubs;
+    for (uint32_t tries = 0; tries < kContSubs && __ballot(want) != 0; ++tries) {
+        const uint64_t m = __ballot(want);
+        const uint32_t sub = (blockIdx.x + tries) % kContSubs;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(counts + (vi * kContSubs + sub) * kHeadStride, (uint32_t)__popcll(m));
+        base = uniform((uint32_t)__shfl((int)base, 0));
+        if (want) {
+            const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (slot < sub_cap) {
+                ContEntry c;
+                suspend_ray(r, c);
+                out[v.cont_base + sub * sub_cap + slot] = c;
+                want = false;
+            }
+        }
+    }
+}
+
+/* Every pixel ray of every view, one lane each (a wave's 64 rays belong to
+ * one view: views' ray ranges are multiples of 64): misses of the box are
+ * written, the others listed for the first pass. */
+__global__ __launch_bounds__(256) void ray_start(const ViewDesc *__restrict__ views, PassArgs a)
+{
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t gw = uniform(g & ~63u);
+    uint32_t vi = 0;
+    while (vi + 1u < a.n_views && views[uniform(vi + 1u)].ray_base <= gw) ++vi;
+    vi = uniform(vi);
+    const ViewDesc &v = views[vi];
+    if (v.kind != kItemView || gw >= v.ray_base + v.n_rays) return;
+    uint32_t pix = 0;
+    F3 V = {0.f, 0.f, 0.f};
+    float t0 = 0.f, t1 = 0.f;
+    const int k = ray_setup(v, g - v.ray_base, pix, V, t0, t1);
+    Ray r;
+    r.phase = PH_IDLE;
+    if (k == RAY_MISS) {
+        r.pix = pix;
+        r.evals = 0;
+        write_miss(v, r);
+    } else if (k == RAY_HIT) {
+        start_hit(v, pix, V, t0, t1, r);
+    }
+    append_cont(v, vi, r, k == RAY_HIT, a.cont_out, a.out_counts);
+}
+
+/* One round for every listed ray, K lanes each (the samples of sample_point
+ * ranks 0..K-1), then: rays whose march ended on a miss are written, rays
+ * that found their opaque sample park for the refinement launch, the others
+ * are listed for the next pass. DRAIN: `rounds` rounds (0: until the march
+ * of every ray of the wave has ended) instead of one. */
+template <int W, int PREC, bool DRAIN>
+__global__ __launch_bounds__(256) void march_pass(const ViewDesc *__restrict__ views, PassArgs a)
+{
+    __shared__ PassStash stash[4];
+    PassStash &st = stash[threadIdx.x >> 6];
+    const uint32_t lane = __lane_id();
+    const uint32_t K = uniform(a.K);
+    const uint32_t per = 64u / K;                 /* rays per wave and round */
+    const uint32_t rank = lane % K;
+    const uint32_t gbase = lane - rank;
+    const uint64_t kmask = K >= 64u ? ~0ull : ((1ull << K) - 1ull);
+    const uint32_t n_waves = gridDim.x * 4u;
+    const uint32_t wid = uniform(blockIdx.x * 4u + threadIdx.x / 64u);
+    const uint32_t pairs = a.n_views * kContSubs;
+    uint32_t chunks = 0;
+    for (uint32_t p = 0; p < pairs; ++p)
+        chunks += (cont_count(views, a.in_counts, kContSubs, p / kContSubs, p % kContSubs) + per - 1u) / per;
+    uint32_t done_evals = 0;
+    for (uint32_t c = wid; c < chunks; c += n_waves) {
+        uint32_t acc = 0, p = 0, cnt = 0;
+        for (; p < pairs; ++p) {
+            cnt = cont_count(views, a.in_counts, kContSubs, p / kContSubs, p % kContSubs);
+            const uint32_t nc = (cnt + per - 1u) / per;
+            if (c < acc + nc) break;
+            acc += nc;
+        }
+        const uint32_t vi = uniform(p / kContSubs);
+        const ViewDesc &v = views[vi];
+        const uint32_t i = (c - acc) * per + lane / K;
+        bool busy = i < cnt;
+        Ray r;
+        r.phase = PH_IDLE;
+        if (busy) resume_ray(a.cont_in[v.cont_base + (p % kContSubs) * (v.cont_cap / kContSubs) + i], r);
+        /* `rounds` rounds (0: until every ray of the wave has ended); a ray
+         * whose march ends leaves its group idle */
+        for (uint32_t round = 0;; ++round) {
+            float lam = NAN;
+            F3 sp = {0.f, 0.f, 0.f};
+            float ts = 0.f;
+            const bool valid = busy && sample_point(v, r, rank, K, sp, ts);
+            if (valid) lam = eval_lambda<W, PREC>(v.e, sp.x, sp.y, sp.z);
+            /* does sample `rank` stop the march (opaque, invalid, a near/far
+             * toggle, or jitter, whose step needs this lambda)? */
+            bool ev = true;
+            if (valid) {
+                const LyapParams &prm = v.prm;
+                const bool near_ = (r.flags & 1u) != 0;
+                const bool toggles = (r.index + rank) > 0 && r.Ndt != r.Fdt &&
+                                     ((lam <= prm.nearThreshold) != near_);
+                ev = !(lam > prm.opaqueThreshold) || toggles || prm.jitter != 0.0f;
+            }
+            const uint64_t evm = (__ballot(ev) >> gbase) & kmask;
+            /* the previous round's combine has read the stash */
+            wave_sync_lds();
+            st.lam[lane] = lam;
+            st.t[lane] = ts;
+            st.x[lane] = sp.x;
+            st.y[lane] = sp.y;
+            st.z[lane] = sp.z;
+            st.ok[lane] = valid ? 1u : 0u;
+            float sc = 0.0f, sa = 0.0f;
+            if (busy && r.index + rank > 0u) {
+                if (lam > v.prm.chaosThreshold) sc = lam;
+                else if (lam > v.prm.opaqueThreshold) sa = lam;
+            }
+            st.ca[lane] = sc;
+            st.aa[lane] = sa;
+            wave_sync_lds();
+            bool finished = false;
+            combine_march<PassStash, 64u>(v, r, evm, st, gbase, K, busy ? (int)PH_MARCH : (int)PH_IDLE,
+                                          rank == 0, finished);
+            if (finished && rank == 0) done_evals += r.evals;
+            /* park the rays that found their opaque sample: sub-region by
+             * block, the next ones while a sub-region is full (the regions
+             * hold twice the view's rays, so one has room) */
+            bool pk = busy && rank == 0 && r.phase == PH_REFINE;
+            for (uint32_t tries = 0; tries < kRefineSubs && __ballot(pk) != 0; ++tries) {
+                const uint64_t pm = __ballot(pk);
+                const uint32_t sub = (blockIdx.x + tries) % kRefineSubs;
+                uint32_t base = 0;
+                if (lane == 0)
+                    base = atomicAdd(a.refine_counts + (vi * kRefineSubs + sub) * kHeadStride,
+                                     (uint32_t)__popcll(pm));
+                base = uniform((uint32_t)__shfl((int)base, 0));
+                if (pk) {
+                    const uint32_t slot = base + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+                    if (slot < v.refine_cap) {
+                        RefineRec q;
+                        q.pix = r.pix;
+                        q.evals = r.evals;
+                        q.Vx = r.V.x;
+                        q.Vy = r.V.y;
+                        q.Vz = r.V.z;
+                        q.lo = r.lo();
+                        q.hi = r.hi();
+                        q.l_hi = r.l_hi();
+                        q.dt = r.dt;
+                        q.a = r.a;
+                        q.c = r.c;
+                        q.l_lo = r.l_lo();
+                        a.refine_out[v.refine_base + sub * v.refine_cap + slot] = q;
+                        pk = false;
+                    }
+                }
+            }
+            busy = busy && r.phase == PH_MARCH;
+            if (!DRAIN || (a.rounds != 0 && round + 1u >= a.rounds) || __ballot(busy) == 0) break;
+        }
+        append_cont(v, vi, r, busy && rank == 0, a.cont_out, a.out_counts);
+    }
+    unsigned long long sum = done_evals;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_down(sum, off);
+    if (lane == 0 && sum) atomicAdd(a.eval_counter, sum);
+}
 
 /* ---- refinement launch -------------------------------------------------- */
 
@@ -1766,6 +2009,7 @@ struct LaunchPlan {
     uint32_t voxel_tile;          /* voxels per claim of a volume block */
     uint32_t budget[3];           /* level budgets chosen by the tuning, 0 = built-in */
     bool direct;
+    bool levels;                  /* the persistent levels only: no march passes */
 };
 
 /* Voxels per work-queue claim of a volume block unless tuned: four waves'
@@ -1825,10 +2069,13 @@ static uint32_t kernel_blocks_per_cu(uint32_t W, int precision, bool margin)
 }
 
 static LaunchPlan plan_launch(const lyap_ctx *ctx, uint64_t rays, const LyapTuning &tune,
-                              uint32_t W, int precision, bool margin, bool direct = false)
+                              uint32_t W, int precision, bool margin, bool direct = false,
+                              bool levels = false)
 {
     LaunchPlan p;
     p.direct = direct;
+    /* the per-wave record describes the persistent levels */
+    p.levels = levels || ctx->debug_wave_log != nullptr;
     const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
     uint32_t per_cu = kernel_blocks_per_cu(W, precision, margin);
     if (tune.waves_per_cu && (tune.waves_per_cu + 3u) / 4u < per_cu) per_cu = (tune.waves_per_cu + 3u) / 4u;
@@ -1932,6 +2179,50 @@ static bool parks(const LaunchPlan &plan, const ViewDesc *views)
            views[0].bisect_steps > 0;
 }
 
+/* March passes (march_pass): the lanes per ray of each pass, at most
+ * kMaxPasses of them; rays still marching after the last pass finish in a
+ * persistent launch. */
+constexpr uint32_t kMaxPasses = 12;
+
+struct PassPlan {
+    uint32_t n = 0;                   /* passes; 0: the persistent levels */
+    uint32_t K[kMaxPasses] = {};
+    uint32_t R[kMaxPasses] = {};      /* rounds of each pass (0: until the rays end) */
+};
+
+/* March passes for small launches of views refined by bisection (the
+ * batched schedule's plan gives four or more lanes per ray): K = 4, 8, 16,
+ * 32, 64, 64 lanes per ray for one round each, then 64 lanes until the rays
+ * end. Measured on the 512x512 frame against other sequences: starting at 8
+ * lanes, fewer or more 64-lane passes, or no draining pass (a persistent
+ * launch finishing the rays) were 1-12 % slower. LYAP_MARCH_PASSES
+ * overrides the sequence ("K" or "KxR" per pass, R rounds, 0 = until the
+ * rays end; "0" switches the passes off). */
+static PassPlan pass_plan(const LaunchPlan &plan, const ViewDesc *views, unsigned n_views)
+{
+    PassPlan pp;
+    if (plan.direct || plan.levels || !parks(plan, views) || plan.lanes_per_ray < 4u) return pp;
+    for (unsigned i = 0; i < n_views; ++i)
+        if (views[i].kind != kItemView) return pp;
+    /* "K" or "KxR" per pass, comma separated */
+    static const char *env = std::getenv("LYAP_MARCH_PASSES");
+    const char *c = env ? env : "4,8,16,32,64,64,64x0";
+    if (c[0] == '0' && c[1] == 0) return pp;
+    while (*c && pp.n < kMaxPasses) {
+        char *e = nullptr;
+        const unsigned long k = std::strtoul(c, &e, 10);
+        if (k == 0 || k > 64 || (k & (k - 1))) return PassPlan{};
+        unsigned long r = 1;
+        if (*e == 'x') r = std::strtoul(e + 1, &e, 10);
+        pp.K[pp.n] = (uint32_t)k;
+        pp.R[pp.n] = (uint32_t)r;
+        ++pp.n;
+        if (*e != ',') break;
+        c = e + 1;
+    }
+    return pp;
+}
+
 /* Prepares `views` for a launch: tile ranges, continuation regions, the
  * common coefficient window; sizes the resources and fills the pinned
  * staging copy. `may_sync` is false while a stream capture is open. */
@@ -1940,7 +2231,8 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
                          bool may_sync, uint32_t *total_tiles)
 {
     uint32_t tiles = 0;
-    uint64_t cap_total = 0, refine_total = 0, list_total = 0;
+    uint64_t cap_total = 0, refine_total = 0, list_total = 0, ray_total = 0;
+    const bool passes = pass_plan(plan, views, n_views).n > 0;
     for (unsigned i = 0; i < n_views; ++i) {
         if (views[i].e.W != W) rebuild_blocks(&views[i].e, W);
         views[i].tile_begin = tiles;
@@ -1960,9 +2252,13 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
         views[i].tile_len = plan.tile_rays;
         views[i].n_tiles = (views[i].n_rays + plan.tile_rays - 1) / plan.tile_rays;
         tiles += views[i].n_tiles;
-        /* continuation region: a quarter of the view's rays, at least 256 */
+        /* continuation region: a quarter of the view's rays, at least 256;
+         * march passes list every ray that meets the box */
         uint32_t cap = views[i].n_rays / 4u;
         if (cap < 256u) cap = 256u;
+        if (passes) cap = views[i].n_rays;
+        views[i].ray_base = (uint32_t)ray_total;
+        ray_total += views[i].n_rays;
         views[i].cont_base = (uint32_t)cap_total;
         views[i].cont_cap = cap;
         cap_total += cap;
@@ -1975,13 +2271,15 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
         views[i].refine_cap = rcap;
         refine_total += (uint64_t)rcap * kRefineSubs;
     }
-    if (cap_total > 0xFFFFFFFFull || refine_total > 0xFFFFFFFFull || list_total > 0xFFFFFFFFull)
+    if (cap_total > 0xFFFFFFFFull || refine_total > 0xFFFFFFFFull || list_total > 0xFFFFFFFFull ||
+        ray_total > 0xFFFFFFFFull)
         return fail(ctx, LYAP_ERR_UNSUPPORTED, "too many rays or voxels in one launch");
     *total_tiles = tiles;
     const size_t table = sizeof(ViewDesc) * n_views;
     int rc = ensure_buffer(ctx, &res.cont_buf, &res.cont_buf_bytes, 2 * cap_total * sizeof(ContEntry), may_sync);
     if (rc) return rc;
-    rc = ensure_buffer(ctx, &res.cont_counts, &res.cont_counts_bytes, 3 * n_views * sizeof(uint32_t), may_sync);
+    rc = ensure_buffer(ctx, &res.cont_counts, &res.cont_counts_bytes,
+                       (size_t)(kMaxPasses + 1) * n_views * kContSubs * kHeadStride * sizeof(uint32_t), may_sync);
     if (rc) return rc;
     if (parks(plan, views)) {
         rc = ensure_buffer(ctx, &res.refine_buf, &res.refine_buf_bytes, refine_total * sizeof(RefineRec),
@@ -2024,6 +2322,116 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
     return LYAP_OK;
 }
 
+/* The refinement launch over one record set, on `stream`. */
+static int launch_refine(lyap_ctx *ctx, const RenderResources &res, unsigned n_views, uint32_t W,
+                         int precision, bool margin)
+{
+    RefineArgs ra{};
+    ra.n_views = n_views;
+    ra.recs = (const RefineRec *)res.refine_buf;
+    ra.counts = (const uint32_t *)res.refine_counts;
+    ra.eval_counter = (unsigned long long *)(res.counters + 2);
+    const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
+    const uint32_t blocks = cus * 8u;
+#define LYAP_LAUNCH(WW, PP, MM)                                                                     \
+    hipLaunchKernelGGL((refine_rays<WW, PP, MM>), dim3(blocks), dim3(256), 0, ctx->stream,          \
+                       (const ViewDesc *)res.view_table, ra)
+    if (W == 16) {
+        if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(16, LYAP_PREC_PLAIN, false);
+        else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, true);
+        else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, false);
+        else LYAP_LAUNCH(16, LYAP_PREC_FP64, false);
+    } else {
+        if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(32, LYAP_PREC_PLAIN, false);
+        else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(32, LYAP_PREC_PRODUCT, true);
+        else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH(32, LYAP_PREC_PRODUCT, false);
+        else LYAP_LAUNCH(32, LYAP_PREC_FP64, false);
+    }
+#undef LYAP_LAUNCH
+    LYAP_HIP(ctx, hipGetLastError());
+    return LYAP_OK;
+}
+
+template <int W, int PREC, bool DRAIN>
+static uint32_t pass_blocks_per_cu()
+{
+    static int cached = -1;
+    if (cached < 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void *>(&march_pass<W, PREC, DRAIN>), 256, 0) != hipSuccess ||
+            n <= 0)
+            n = 2;
+        cached = n;
+    }
+    return (uint32_t)cached;
+}
+
+/* Enqueues ray_start and the march passes: ray_start lists the pixels' rays
+ * that meet the box under counts[0], pass p reads the rays listed under
+ * counts[p] and lists the rays still marching under counts[p + 1] (the two
+ * buffers alternate). */
+static int enqueue_passes(lyap_ctx *ctx, RenderResources &res, const ViewDesc *views, unsigned n_views,
+                          const PassPlan &pp, uint32_t W, int precision)
+{
+    uint64_t cap_total = 0, rays = 0;
+    for (unsigned i = 0; i < n_views; ++i) {
+        cap_total += views[i].cont_cap;
+        rays += views[i].n_rays;
+    }
+    ContEntry *buf[2] = {(ContEntry *)res.cont_buf, (ContEntry *)res.cont_buf + cap_total};
+    uint32_t *counts = (uint32_t *)res.cont_counts;
+    const uint32_t pass_words = n_views * kContSubs * kHeadStride;
+    const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
+    PassArgs a{};
+    a.n_views = n_views;
+    a.refine_out = (RefineRec *)res.refine_buf;
+    a.refine_counts = (uint32_t *)res.refine_counts;
+    a.eval_counter = (unsigned long long *)(res.counters + 2);
+    a.cont_out = buf[0];
+    a.out_counts = counts;
+    int rc = timing_mark(ctx);
+    if (rc) return rc;
+    hipLaunchKernelGGL(ray_start, dim3((uint32_t)((rays + 255u) / 256u)), dim3(256), 0, ctx->stream,
+                       (const ViewDesc *)res.view_table, a);
+    LYAP_HIP(ctx, hipGetLastError());
+    rc = timing_mark(ctx);
+    if (rc) return rc;
+    for (uint32_t p = 0; p < pp.n; ++p) {
+        a.K = pp.K[p];
+        a.rounds = pp.R[p];
+        a.cont_in = buf[p & 1u];
+        a.in_counts = counts + p * pass_words;
+        a.cont_out = buf[(p + 1u) & 1u];
+        a.out_counts = counts + (p + 1u) * pass_words;
+        rc = timing_mark(ctx);
+        if (rc) return rc;
+#define LYAP_LAUNCH(WW, PP, DD)                                                                      \
+    hipLaunchKernelGGL((march_pass<WW, PP, DD>), dim3(cus * pass_blocks_per_cu<WW, PP, DD>()), dim3(256), \
+                       0, ctx->stream, (const ViewDesc *)res.view_table, a)
+#define LYAP_LAUNCH2(WW, PP)                                                                         \
+    do {                                                                                             \
+        if (a.rounds == 1u) LYAP_LAUNCH(WW, PP, false);                                              \
+        else LYAP_LAUNCH(WW, PP, true);                                                              \
+    } while (0)
+        if (W == 16) {
+            if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH2(16, LYAP_PREC_PLAIN);
+            else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH2(16, LYAP_PREC_PRODUCT);
+            else LYAP_LAUNCH2(16, LYAP_PREC_FP64);
+        } else {
+            if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH2(32, LYAP_PREC_PLAIN);
+            else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH2(32, LYAP_PREC_PRODUCT);
+            else LYAP_LAUNCH2(32, LYAP_PREC_FP64);
+        }
+#undef LYAP_LAUNCH2
+#undef LYAP_LAUNCH
+        LYAP_HIP(ctx, hipGetLastError());
+        rc = timing_mark(ctx);
+        if (rc) return rc;
+    }
+    return LYAP_OK;
+}
+
 /* Enqueues the upload and the level launches on the context's stream. No
  * call here synchronises, so it may be recorded into a graph. */
 static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *views,
@@ -2038,8 +2446,9 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
     ZeroRegions z{};
     z.p[0] = res.counters;
     z.n[0] = kCounterWords;
+    const PassPlan pp = pass_plan(plan, views, n_views);
     z.p[1] = (uint32_t *)res.cont_counts;
-    z.n[1] = 3 * n_views;
+    z.n[1] = pp.n ? (pp.n + 1u) * n_views * kContSubs * kHeadStride : 3u * n_views;
     z.p[2] = park ? (uint32_t *)res.refine_counts : nullptr;
     z.n[2] = park ? n_views * kRefineSubs * kHeadStride : 0u;
     z.p[3] = (uint32_t *)res.vox_counts;
@@ -2073,8 +2482,18 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
     ContEntry *out_buf[4] = {bufA, bufB, bufA, nullptr};
     const ContEntry *in_buf[4] = {nullptr, bufA, bufB, bufA};
     /* volume blocks finish at level 0; later levels carry rays only */
-    const uint32_t levels = (plan.direct || !has_rays) ? 1u : 4u;
-    for (uint32_t level = 0; level < levels; ++level) {
+    uint32_t levels = (plan.direct || !has_rays) ? 1u : 4u;
+    uint32_t first_level = 0;
+    if (pp.n > 0) {
+        rc = enqueue_passes(ctx, res, views, n_views, pp, W, precision);
+        if (rc) return rc;
+        /* the rays still marching after the last pass finish in one
+         * persistent launch (a later level: no budget, handed-over input) */
+        in_buf[3] = (pp.n & 1u) ? bufB : bufA;
+        /* a last pass that runs its rays to the end leaves none */
+        first_level = pp.R[pp.n - 1] == 0 ? levels : 3u;
+    }
+    for (uint32_t level = first_level; level < levels; ++level) {
         RenderArgs a{};
         a.n_views = n_views;
         a.total_tiles = total_tiles;
@@ -2089,7 +2508,10 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         a.direct = plan.direct ? 1u : 0u;
         a.resident_lanes = plan.resident_lanes;
         a.cont_in = in_buf[level];
-        a.in_counts = level ? counts + (level - 1) * n_views : nullptr;
+        a.in_counts = level ? (pp.n ? counts + pp.n * n_views * kContSubs * kHeadStride
+                                    : counts + (level - 1) * n_views)
+                            : nullptr;
+        a.in_subs = pp.n ? kContSubs : 0u;
         a.cont_out = out_buf[level];
         a.out_counts = level < 3 ? counts + level * n_views : nullptr;
         a.refine_out = park ? (RefineRec *)res.refine_buf : nullptr;
@@ -2137,30 +2559,10 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         }
     }
     if (park && has_rays) {
-        RefineArgs ra{};
-        ra.n_views = n_views;
-        ra.recs = (const RefineRec *)res.refine_buf;
-        ra.counts = (const uint32_t *)res.refine_counts;
-        ra.eval_counter = (unsigned long long *)(res.counters + 2);
-        const uint32_t blocks = cus * 8u;
         rc = timing_mark(ctx);
         if (rc) return rc;
-#define LYAP_LAUNCH(WW, PP, MM)                                                       \
-    hipLaunchKernelGGL((refine_rays<WW, PP, MM>), dim3(blocks), dim3(256), 0, ctx->stream, \
-                       (const ViewDesc *)res.view_table, ra)
-        if (W == 16) {
-            if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(16, LYAP_PREC_PLAIN, false);
-            else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, true);
-            else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH(16, LYAP_PREC_PRODUCT, false);
-            else LYAP_LAUNCH(16, LYAP_PREC_FP64, false);
-        } else {
-            if (precision == LYAP_PREC_PLAIN) LYAP_LAUNCH(32, LYAP_PREC_PLAIN, false);
-            else if (precision == LYAP_PREC_PRODUCT && margin) LYAP_LAUNCH(32, LYAP_PREC_PRODUCT, true);
-            else if (precision == LYAP_PREC_PRODUCT) LYAP_LAUNCH(32, LYAP_PREC_PRODUCT, false);
-            else LYAP_LAUNCH(32, LYAP_PREC_FP64, false);
-        }
-#undef LYAP_LAUNCH
-        LYAP_HIP(ctx, hipGetLastError());
+        rc = launch_refine(ctx, res, n_views, W, precision, margin);
+        if (rc) return rc;
         rc = timing_mark(ctx);
         if (rc) return rc;
     }
@@ -2198,7 +2600,8 @@ static int check_mode(lyap_ctx *ctx, const LyapRenderMode *mode_in, LyapRenderMo
         return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "bisect_steps outside 0..64");
     if (mode.lanes_per_ray > 64)
         return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "lanes_per_ray above 64");
-    if (mode.schedule != LYAP_SCHEDULE_BATCHED && mode.schedule != LYAP_SCHEDULE_DIRECT)
+    if (mode.schedule != LYAP_SCHEDULE_BATCHED && mode.schedule != LYAP_SCHEDULE_DIRECT &&
+        mode.schedule != LYAP_SCHEDULE_LEVELS)
         return fail(ctx, LYAP_ERR_INVALID_ARGUMENT, "unknown schedule %d", mode.schedule);
     return LYAP_OK;
 }
@@ -2281,7 +2684,8 @@ static int render_views(lyap_ctx *ctx, std::vector<ViewDesc> &views, uint32_t W,
     LyapTuning t = tune;
     if (!t.lanes_per_ray && mode.lanes_per_ray) t.lanes_per_ray = mode.lanes_per_ray;
     const LaunchPlan plan = plan_launch(ctx, rays, t, W, mode.precision, margin,
-                                        mode.schedule == LYAP_SCHEDULE_DIRECT);
+                                        mode.schedule == LYAP_SCHEDULE_DIRECT,
+                                        mode.schedule == LYAP_SCHEDULE_LEVELS);
     RenderResources &res = ctx_resources(ctx);
     uint32_t total_tiles = 0;
     int rc = prepare_views(ctx, res, views.data(), (unsigned)views.size(), plan, W, true, &total_tiles);
@@ -2381,7 +2785,8 @@ int render_slot_launch(lyap_ctx *ctx, RenderSlot *slot, const LyapView *in,
     LyapTuning t = tune;
     if (!t.lanes_per_ray && mode.lanes_per_ray) t.lanes_per_ray = mode.lanes_per_ray;
     const LaunchPlan plan = plan_launch(ctx, rays, t, W, mode.precision, margin,
-                                        mode.schedule == LYAP_SCHEDULE_DIRECT);
+                                        mode.schedule == LYAP_SCHEDULE_DIRECT,
+                                        mode.schedule == LYAP_SCHEDULE_LEVELS);
     uint32_t total_tiles = 0;
     const unsigned items = (unsigned)views.size();
     rc = prepare_views(ctx, slot->res, views.data(), items, plan, W, false, &total_tiles);
@@ -2726,7 +3131,8 @@ extern "C" int lyap_frame_capture(lyap_ctx *ctx, const LyapView *views, unsigned
     for (auto &v : vd) rays += v.n_rays;
     const bool margin = mode.fp64_margin > 0.0f && mode.refine == LYAP_REFINE_BISECT;
     const LaunchPlan plan = plan_launch(ctx, rays, merged_tuning(ctx, tuning), W, mode.precision, margin,
-                                        mode.schedule == LYAP_SCHEDULE_DIRECT);
+                                        mode.schedule == LYAP_SCHEDULE_DIRECT,
+                                        mode.schedule == LYAP_SCHEDULE_LEVELS);
     lyap_frame *f = new lyap_frame();
     f->ctx = ctx;
     uint32_t total_tiles = 0;

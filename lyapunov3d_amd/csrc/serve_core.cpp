@@ -62,7 +62,8 @@ int check_config(const LyapServerConfig *in, LyapServerConfig *out, std::string 
         bad = "chunk_rays must be a multiple of 64";
     } else if (cfg.mode.refine != LYAP_REFINE_WALK && cfg.mode.refine != LYAP_REFINE_BISECT) {
         bad = "unknown refine mode";
-    } else if (cfg.mode.schedule != LYAP_SCHEDULE_BATCHED && cfg.mode.schedule != LYAP_SCHEDULE_DIRECT) {
+    } else if (cfg.mode.schedule != LYAP_SCHEDULE_BATCHED && cfg.mode.schedule != LYAP_SCHEDULE_DIRECT &&
+               cfg.mode.schedule != LYAP_SCHEDULE_LEVELS) {
         bad = "unknown schedule";
     } else if (cfg.mode.bisect_steps < 0 || cfg.mode.bisect_steps > 64 || cfg.mode.lanes_per_ray > 64) {
         bad = "bad render mode";

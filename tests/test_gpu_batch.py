@@ -230,6 +230,60 @@ def test_direct_schedule_same_bytes(ctx, prec, margin):
         same(g.fetch(), one.fetch())
 
 
+@pytest.mark.parametrize("prec,margin", [(lyap.PREC_PLAIN, 0.0), (lyap.PREC_PRODUCT, 0.0),
+                                         (lyap.PREC_FP64, 0.0), (lyap.PREC_PRODUCT, 0.01)])
+def test_march_passes_same_bytes(ctx, prec, margin):
+    """A small launch marches in passes (a set-up launch, then launches of
+    one round each with 4 to 64 lanes per ray, the last running its rays to
+    the end); the persistent levels (LYAP_SCHEDULE_LEVELS) and the
+    reference's shape give the same bytes and evaluation counts, view by
+    view (the 20-letter sequence takes the 32-slot window) and as one batch."""
+    views, lights, n_lights = small_batch(6, seed=5)
+    ld = device_lights(lights, n_lights)
+    passes = W.c2_mode(precision=prec, fp64_margin=margin)
+    levels = W.c2_mode(precision=prec, fp64_margin=margin)
+    levels.schedule = lyap.SCHEDULE_LEVELS
+    direct = W.c2_mode(precision=prec, fp64_margin=margin)
+    direct.schedule = lyap.SCHEDULE_DIRECT
+    for v in views:
+        out, ev = [], []
+        for m in (passes, levels, direct):
+            b = RenderBuffers(v["w"], v["h"])
+            b.render(ctx, v["cam"], v["prm"], v["seq"], ld, n_lights, m)
+            ctx.sync()
+            ev.append(ctx.last_evals())
+            out.append(b.fetch())
+        assert ev[0] == ev[1] == ev[2]
+        same(out[0], out[1])
+        same(out[0], out[2])
+    ref = []
+    for m in (passes, levels):
+        bufs = alloc(views)
+        ctx.render_batch(as_api(views, bufs, ld, n_lights), m)
+        ctx.sync()
+        ref.append((ctx.last_evals(), [b.fetch() for b in bufs]))
+    assert ref[0][0] == ref[1][0]
+    for a, b in zip(ref[0][1], ref[1][1]):
+        same(a, b)
+
+
+def test_march_passes_benched_frame(ctx):
+    """The 512x512 frame in the benchmarked mode (product form, no FP64
+    margin): passes and persistent levels give the same bytes."""
+    cam, prm, seq, lights, n_lights = W.render_config(512, 512)
+    ld = device_lights(lights, n_lights)
+    out = []
+    for sched in (lyap.SCHEDULE_BATCHED, lyap.SCHEDULE_LEVELS):
+        m = W.c2_mode(precision=lyap.PREC_PRODUCT, fp64_margin=0.0)
+        m.schedule = sched
+        b = RenderBuffers(512, 512)
+        b.render(ctx, cam, prm, seq, ld, n_lights, m)
+        ctx.sync()
+        out.append((ctx.last_evals(), b.fetch()))
+    assert out[0][0] == out[1][0]
+    same(out[0][1], out[1][1])
+
+
 def test_bad_schedule(ctx):
     views, lights, n_lights = small_batch(1)
     ld = device_lights(lights, n_lights)

@@ -91,14 +91,26 @@ def test_timing_counts_the_level_launches(ctx):
     ld = device_lights(lights, n_lights)
     bufs = RenderBuffers(128, 128)
     mode = W.c2_mode(fp64_margin=0.0)
+    # the persistent schedule: four march levels and the refinement launch
+    lv = W.c2_mode(fp64_margin=0.0)
+    lv.schedule = lyap.SCHEDULE_LEVELS
+    ctx.timing(True)
+    try:
+        bufs.render(ctx, cam, prm, seq, ld, n_lights, lv)
+        ctx.sync()
+        assert ctx.timing_collect()[1] == 5
+    finally:
+        ctx.timing(False)
+    # a small frame marches in passes: the set-up launch, seven passes and
+    # the refinement launch
     ctx.timing(True)
     try:
         for _ in range(3):
             bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
         ctx.sync()
         ms, launches = ctx.timing_collect()
-        # four march levels and the refinement launch per frame
-        assert launches == 3 * 5 and ms > 0.0
+        per_frame = 9
+        assert launches == 3 * per_frame and ms > 0.0
         # collecting resets the sums
         ms2, launches2 = ctx.timing_collect()
         assert launches2 == 0 and ms2 == 0.0
@@ -113,7 +125,7 @@ def test_timing_counts_the_level_launches(ctx):
         ctx.timing(False)
         bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
         ctx.sync()
-        assert ctx.timing_collect()[1] == 2 * 5
+        assert ctx.timing_collect()[1] == 2 * per_frame
     finally:
         ctx.timing(False)
     # direct schedule: one launch per frame
