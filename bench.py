@@ -823,9 +823,10 @@ def main():
 
     rays = w * h
     frame_ms = float(per.mean())
-    # the render kernels: four march level launches and the refinement
-    # launch per frame; algorithmic FLOP of a frame over the summed duration
-    # of its launches (HIP events on the library stream around each launch)
+    # the render kernels: the march (a set-up launch and seven one-round
+    # passes on this frame, or four persistent levels) and the refinement
+    # launch; algorithmic FLOP of a frame over the summed duration of its
+    # launches (HIP events on the library stream around each launch)
     launch_ms = kernel_ms / max(launches, 1)
     timed_frames = len(range(0, args.steps, 4))
     launches_per_frame = launches / timed_frames
@@ -865,14 +866,15 @@ def main():
             "traffic": traffic,
             "traffic_source": traffic_src,
             "flop_per_lambda_eval": flops(prm),
-            "kernel": "render_persistent (4 levels) + refine_rays",
+            "kernel": ("ray_start + march_pass x 7 + refine_rays" if round(launches_per_frame) == 9
+                       else "render_persistent (4 levels) + refine_rays"),
             "launches_per_frame": launches_per_frame,
             "avg_launch_us": launch_ms * 1e3,
             "flop_per_launch": evals_per_frame * flops(prm) / launches_per_frame,
             "note": "achieved = algorithmic FLOP per launch (lambda evaluations x (3 settle + 6 accum)) / "
-                    "mean launch duration from HIP events on the library stream, over the frame's five "
-                    "launches; traffic = HBM bytes per launch (mean of the five) from rocprofv3 "
-                    "FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy",
+                    "mean launch duration from HIP events on the library stream, over the frame's "
+                    f"{launches_per_frame:.0f} launches; traffic = HBM bytes per launch (mean over the "
+                    "frame's launches) from rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB copy",
         },
     }
     ctx.set_tuning(None)          # the other workloads tune for themselves or use built-in shapes
