@@ -2262,10 +2262,10 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
         views[i].cont_base = (uint32_t)cap_total;
         views[i].cont_cap = cap;
         cap_total += cap;
-        /* refinement records: each sub-region holds a quarter of the view's
-         * rays, so one XCD may park twice its share before a ray has to be
-         * refined in place */
-        uint32_t rcap = (views[i].n_rays + 3u) / 4u;
+        /* refinement records: each sub-region holds twice its share of the
+         * view's rays before a ray has to be refined in place (or, in the
+         * march passes, go to the next sub-region) */
+        uint32_t rcap = (2u * views[i].n_rays + kRefineSubs - 1u) / kRefineSubs;
         if (rcap < 64u) rcap = 64u;
         views[i].refine_base = (uint32_t)refine_total;
         views[i].refine_cap = rcap;
