@@ -1588,22 +1588,29 @@ __device__ __forceinline__ void append_cont(const ViewDesc &v, uint32_t vi, cons
     }
 }
 
-/* Every pixel ray of every view, one lane each (a wave's 64 rays belong to
- * one view: views' ray ranges are multiples of 64): misses of the box are
- * written, the others listed for the first pass. */
+/* Every pixel ray of every view, one lane each (a block's 256 rays belong
+ * to one view: views start at multiples of 256 in this grid): misses of the
+ * box are written, the others listed for the first pass with one atomic per
+ * block on the sub-list of its XCD (rays that do not fit go on to the next
+ * sub-lists, as append_cont). */
 __global__ __launch_bounds__(256) void ray_start(const ViewDesc *__restrict__ views, PassArgs a)
 {
+    __shared__ uint32_t wave_hits[4];
+    __shared__ uint32_t block_base;
     const uint32_t g = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t gw = uniform(g & ~63u);
+    const uint32_t gb = uniform(blockIdx.x * 256u);
     uint32_t vi = 0;
-    while (vi + 1u < a.n_views && views[uniform(vi + 1u)].ray_base <= gw) ++vi;
+    while (vi + 1u < a.n_views && views[uniform(vi + 1u)].ray_base <= gb) ++vi;
     vi = uniform(vi);
     const ViewDesc &v = views[vi];
-    if (v.kind != kItemView || gw >= v.ray_base + v.n_rays) return;
+    if (v.kind != kItemView || gb >= v.ray_base + v.n_rays) return;   /* the whole block */
+    const uint32_t lane = __lane_id();
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t id = g - v.ray_base;
     uint32_t pix = 0;
     F3 V = {0.f, 0.f, 0.f};
     float t0 = 0.f, t1 = 0.f;
-    const int k = ray_setup(v, g - v.ray_base, pix, V, t0, t1);
+    const int k = id < v.n_rays ? ray_setup(v, id, pix, V, t0, t1) : RAY_PAD;
     Ray r;
     r.phase = PH_IDLE;
     if (k == RAY_MISS) {
@@ -1613,7 +1620,30 @@ __global__ __launch_bounds__(256) void ray_start(const ViewDesc *__restrict__ vi
     } else if (k == RAY_HIT) {
         start_hit(v, pix, V, t0, t1, r);
     }
-    append_cont(v, vi, r, k == RAY_HIT, a.cont_out, a.out_counts);
+    bool want = k == RAY_HIT;
+    const uint64_t m = __ballot(want);
+    if (lane == 0) wave_hits[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = wave_hits[0] + wave_hits[1] + wave_hits[2] + wave_hits[3];
+        block_base = total ? atomicAdd(a.out_counts + (vi * kContSubs + blockIdx.x % kContSubs) * kHeadStride,
+                                       total)
+                           : 0u;
+    }
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t k2 = 0; k2 < w; ++k2) before += wave_hits[k2];
+    if (want) {
+        const uint32_t sub_cap = v.cont_cap / kContSubs;
+        const uint32_t slot = block_base + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (slot < sub_cap) {
+            ContEntry c;
+            suspend_ray(r, c);
+            a.cont_out[v.cont_base + (blockIdx.x % kContSubs) * sub_cap + slot] = c;
+            want = false;
+        }
+    }
+    append_cont(v, vi, r, want, a.cont_out, a.out_counts);
 }
 
 /* One round for every listed ray, K lanes each (the samples of sample_point
@@ -2257,8 +2287,9 @@ static int prepare_views(lyap_ctx *ctx, RenderResources &res, ViewDesc *views,
         uint32_t cap = views[i].n_rays / 4u;
         if (cap < 256u) cap = 256u;
         if (passes) cap = views[i].n_rays;
+        /* ray_start's grid: every view starts a block of its own */
         views[i].ray_base = (uint32_t)ray_total;
-        ray_total += views[i].n_rays;
+        ray_total += (views[i].n_rays + 255u) / 256u * 256u;
         views[i].cont_base = (uint32_t)cap_total;
         views[i].cont_cap = cap;
         cap_total += cap;
@@ -2377,7 +2408,7 @@ static int enqueue_passes(lyap_ctx *ctx, RenderResources &res, const ViewDesc *v
     uint64_t cap_total = 0, rays = 0;
     for (unsigned i = 0; i < n_views; ++i) {
         cap_total += views[i].cont_cap;
-        rays += views[i].n_rays;
+        rays += (views[i].n_rays + 255u) / 256u * 256u;
     }
     ContEntry *buf[2] = {(ContEntry *)res.cont_buf, (ContEntry *)res.cont_buf + cap_total};
     uint32_t *counts = (uint32_t *)res.cont_counts;
