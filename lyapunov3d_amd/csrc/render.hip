@@ -76,8 +76,9 @@ struct ViewDesc {
 constexpr uint32_t kItemView = 0;
 constexpr uint32_t kItemVolume = 1;
 
-/* March state of a ray handed to the next launch (96 bytes). */
-struct ContEntry {
+/* March state of a ray handed to the next launch (96 bytes; 16-byte
+ * aligned, so a record moves in six 16-byte loads or stores). */
+struct alignas(16) ContEntry {
     uint32_t pix, evals, index, flags;
     float Vx, Vy, Vz, Px, Py, Pz;
     float t, t1, dt, Fdt, Ndt, last_step;
@@ -92,7 +93,7 @@ static_assert(sizeof(ContEntry) == 96, "continuation record size");
  * launch: the bisection interval [lo, hi] with lambda at both ends, the
  * march step (the normal offset of the reference's method), the march sums
  * and the evaluations so far (48 bytes). */
-struct RefineRec {
+struct alignas(16) RefineRec {
     uint32_t pix, evals;
     float Vx, Vy, Vz, lo, hi, l_hi, dt, a, c, l_lo;
 };
