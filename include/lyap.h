@@ -224,8 +224,8 @@ typedef struct LyapRenderMode {
 /* How rays are spread over lanes. BATCHED: the persistent work queue with
  * lane groups per ray, refill of finished lanes and continuation launches;
  * with bisection, rays whose march found its opaque sample are refined in a
- * launch of their own, two lanes per ray. A small launch of views refined by
- * bisection (fewer rays than about two per resident lane) marches in passes
+ * launch of their own, two lanes per ray. A small launch of up to 16 views
+ * refined by bisection (fewer rays than about two per resident lane) marches in passes
  * instead: a set-up launch lists the rays that meet the box, then launches
  * of one round each give every listed ray K lanes (4, 8, 16, 32, then 64
  * samples at once), the last running its rays to the end of their march.

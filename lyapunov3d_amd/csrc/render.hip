@@ -2232,7 +2232,10 @@ struct PassPlan {
 static PassPlan pass_plan(const LaunchPlan &plan, const ViewDesc *views, unsigned n_views)
 {
     PassPlan pp;
-    if (plan.direct || plan.levels || !parks(plan, views) || plan.lanes_per_ray < 4u) return pp;
+    /* a pass finds each chunk's view and sub-list by a scan over them, so
+     * launches of many views keep the persistent levels */
+    if (plan.direct || plan.levels || !parks(plan, views) || plan.lanes_per_ray < 4u || n_views > 16u)
+        return pp;
     for (unsigned i = 0; i < n_views; ++i)
         if (views[i].kind != kItemView) return pp;
     /* "K" or "KxR" per pass, comma separated */
