@@ -795,7 +795,7 @@ def main():
     ctx.timing_collect()                      # start from an empty launch record
     # Every frame does its full work; a sample of them also carries events,
     # each of which costs the device a little: every fourth frame has its
-    # level launches timed by the library (the roofline's launch durations),
+    # launches timed by the library (the roofline's launch durations),
     # and another in four is bracketed by events on the library stream (the
     # per-frame percentiles). The throughput is the wall clock over all.
     sampled = [i for i in range(args.steps) if i % 4 == 2] or [0]

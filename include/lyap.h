@@ -374,8 +374,8 @@ int lyap_set_tuning(lyap_ctx *ctx, const LyapTuning *tuning);
 int lyap_get_tuning(lyap_ctx *ctx, LyapTuning *out);
 
 /* A captured frame: the batch upload and the frame's launches (the march
- * levels and the refinement) recorded once as a HIP graph and replayed. The
- * views' device buffers must stay valid. */
+ * passes or levels, and the refinement) recorded once as a HIP graph and
+ * replayed. The views' device buffers must stay valid. */
 typedef struct lyap_frame lyap_frame;
 int lyap_frame_capture(lyap_ctx *ctx, const LyapView *views, unsigned n_views,
                        const LyapRenderMode *mode, const LyapTuning *tuning,
