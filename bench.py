@@ -792,34 +792,35 @@ def main():
     evals_per_frame = ctx.last_evals()
 
     torch.cuda.synchronize()
-    ctx.timing_collect()                      # start from an empty launch record
-    # Every frame does its full work; a sample of them also carries events,
-    # each of which costs the device a little: every fourth frame has its
-    # launches timed by the library (the roofline's launch durations),
-    # and another in four is bracketed by events on the library stream (the
-    # per-frame percentiles). The throughput is the wall clock over all.
+    # Every frame does its full work; one in four is bracketed by two events
+    # on the library stream (the per-frame percentiles). The throughput is the
+    # wall clock over all of them. The roofline's launch durations come from a
+    # sample of frames after the timed region, with the library's events
+    # around each launch (each event costs the device a few microseconds, so
+    # they stay out of the timed frames).
     sampled = [i for i in range(args.steps) if i % 4 == 2] or [0]
     starts = {i: torch.cuda.Event(enable_timing=True) for i in sampled}
     ends = {i: torch.cuda.Event(enable_timing=True) for i in sampled}
     t0 = time.perf_counter()
     for i in range(args.steps):
-        timed = i % 4 == 0
-        if timed:
-            ctx.timing(True)
         if i in starts:
             starts[i].record(timer.stream)
         bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
         if i in ends:
             ends[i].record(timer.stream)
-        if timed:
-            ctx.timing(False)
     ctx.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms, launches = ctx.timing_collect()
     log(f"timed {args.steps} frames in {wall:.3f} s")
     per = np.array([starts[i].elapsed_time(ends[i]) for i in sampled if i < args.steps])
     ms = float(wall * 1000.0 / args.steps)
+    timed_frames = max(10, args.steps // 4)
+    ctx.timing_collect()                      # start from an empty launch record
+    ctx.timing(True)
+    for _ in range(timed_frames):
+        bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
+    ctx.timing(False)
+    kernel_ms, launches = ctx.timing_collect()
 
     rays = w * h
     frame_ms = float(per.mean())
@@ -828,7 +829,6 @@ def main():
     # launch; algorithmic FLOP of a frame over the summed duration of its
     # launches (HIP events on the library stream around each launch)
     launch_ms = kernel_ms / max(launches, 1)
-    timed_frames = len(range(0, args.steps, 4))
     launches_per_frame = launches / timed_frames
     achieved = evals_per_frame * flops(prm) / launches_per_frame / (launch_ms * 1e-3) / 1e12
     traffic, traffic_src = profiled_traffic("render_frame")
