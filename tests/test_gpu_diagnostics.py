@@ -6,6 +6,8 @@ ray meets the box (those, and only those, take at least one evaluation), each
 later launch starts a subset of the rays the previous one did, and every
 wave's clock split stays within its own run time.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -109,7 +111,7 @@ def test_timing_counts_the_level_launches(ctx):
             bufs.render(ctx, cam, prm, seq, ld, n_lights, mode)
         ctx.sync()
         ms, launches = ctx.timing_collect()
-        per_frame = 9
+        per_frame = 5 if os.environ.get("LYAP_MARCH_PASSES") == "0" else 9
         assert launches == 3 * per_frame and ms > 0.0
         # collecting resets the sums
         ms2, launches2 = ctx.timing_collect()
