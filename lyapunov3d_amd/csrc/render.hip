@@ -131,9 +131,7 @@ struct RenderArgs {
     uint32_t resident_lanes;
     uint32_t direct;           /* one lane per ray, no refill until the wave is idle */
     const ContEntry *cont_in;
-    const uint32_t *in_counts;         /* per view, or per view and sub-list (in_subs) */
-    uint32_t in_subs;                  /* 0: one input list per view; kContSubs: the
-                                          march passes' per-XCD sub-lists */
+    const uint32_t *in_counts;         /* per view */
     ContEntry *cont_out;
     uint32_t *out_counts;              /* per view */
     /* bisection rays park here when their march ends (0: refine in place);
@@ -941,14 +939,6 @@ __device__ __forceinline__ uint32_t uniform(uint32_t x)
     return __builtin_amdgcn_readfirstlane(x);
 }
 
-/* Listed rays of view v's sub-list s (subs 0: the view's one list). */
-__device__ __forceinline__ uint32_t cont_count(const ViewDesc *__restrict__ views, const uint32_t *counts,
-                                               uint32_t subs, uint32_t v, uint32_t s)
-{
-    if (subs == 0) return uniform(min(counts[v], views[uniform(v)].cont_cap));
-    return uniform(min(counts[(v * kContSubs + s) * kHeadStride], views[uniform(v)].cont_cap / kContSubs));
-}
-
 /* Rays [first, last) of view `view` for tile t. Level 0 tiles cover the
  * views' pixel blocks; later levels cover each view's handed-over rays,
  * `tile` entries per tile. Views are few: scalar scans. Returns false when t
@@ -970,21 +960,16 @@ __device__ __forceinline__ bool tile_lookup(const ViewDesc *__restrict__ views,
         return true;
     }
     uint32_t acc = 0;
-    const uint32_t ns = a.in_subs ? kContSubs : 1u;
     for (uint32_t v = 0; v < a.n_views; ++v) {
-        const uint32_t sub_cap = a.in_subs ? uniform(views[uniform(v)].cont_cap / kContSubs) : 0u;
-        for (uint32_t s = 0; s < ns; ++s) {
-            const uint32_t cnt = cont_count(views, a.in_counts, a.in_subs, v, s);
-            const uint32_t nt = (cnt + tile - 1) / tile;
-            if (t < acc + nt) {
-                /* positions within the view's region: sub-list s starts at s * sub_cap */
-                view = uniform(v);
-                first = uniform(s * sub_cap + (t - acc) * tile);
-                last = uniform(min(first + tile, s * sub_cap + cnt));
-                return true;
-            }
-            acc += nt;
+        const uint32_t cnt = uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+        const uint32_t nt = (cnt + tile - 1) / tile;
+        if (t < acc + nt) {
+            view = uniform(v);
+            first = uniform((t - acc) * tile);
+            last = uniform(min(first + tile, cnt));
+            return true;
         }
+        acc += nt;
     }
     return false;
 }
@@ -1118,8 +1103,7 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
          * ray and round as lanes would be if each took one */
         uint32_t total = 0;
         for (uint32_t v = 0; v < a.n_views; ++v)
-            for (uint32_t s = 0; s < (a.in_subs ? kContSubs : 1u); ++s)
-                total += cont_count(views, a.in_counts, a.in_subs, v, s);
+            total += uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
         if (total == 0) return;
         K0 = 1;
         while (K0 < 64u && (uint64_t)a.resident_lanes >= (uint64_t)total * (2u * K0)) K0 *= 2;
@@ -1135,9 +1119,10 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
     uint32_t level_tiles = a.total_tiles;
     if (a.level > 0) {
         level_tiles = 0;
-        for (uint32_t v = 0; v < a.n_views; ++v)
-            for (uint32_t s = 0; s < (a.in_subs ? kContSubs : 1u); ++s)
-                level_tiles += (cont_count(views, a.in_counts, a.in_subs, v, s) + tile - 1u) / tile;
+        for (uint32_t v = 0; v < a.n_views; ++v) {
+            const uint32_t cnt = uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+            level_tiles += (cnt + tile - 1u) / tile;
+        }
     }
     uint32_t K = K0;                  /* lanes per ray; wave-uniform */
     const unsigned long long t_start = a.wave_log ? clock64() : 0ull;
@@ -1541,12 +1526,56 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
  * the rays up in the first pass instead, every lane of a group the same ray,
  * raised the pass kernel's registers and made the frame 8 % slower.) */
 
+/* A marching ray between passes (64 bytes). The passes refine by bisection,
+ * where a ray's position is always C + V t, its step Fdt or Fdt /
+ * nearMultiplier by the near flag, and both were formed by those same
+ * expressions: the record keeps t and Fdt and the resumed ray recomputes P,
+ * Ndt and dt bit for bit. */
+struct alignas(16) PassRec {
+    uint32_t pix, evals;
+    uint32_t index_flags;              /* march sample index (< 2^24), flags in the top byte */
+    uint32_t m;
+    float Vx, Vy, Vz, t, t1, Fdt, last_step, a, c, l, tb, dtb;
+};
+static_assert(sizeof(PassRec) == 64, "pass record size");
+
+__device__ __forceinline__ void store_pass(const Ray &r, PassRec &q)
+{
+    q.pix = r.pix;
+    q.evals = r.evals;
+    q.index_flags = (r.index & 0x00FFFFFFu) | (r.flags << 24);
+    q.m = r.m;
+    q.Vx = r.V.x; q.Vy = r.V.y; q.Vz = r.V.z;
+    q.t = r.t; q.t1 = r.t1; q.Fdt = r.Fdt; q.last_step = r.last_step;
+    q.a = r.a; q.c = r.c; q.l = r.l;
+    q.tb = r.tb; q.dtb = r.dtb;
+}
+
+__device__ __forceinline__ void load_pass(const ViewDesc &v, const PassRec &q, Ray &r)
+{
+    r.pix = q.pix;
+    r.evals = q.evals;
+    r.index = q.index_flags & 0x00FFFFFFu;
+    r.flags = q.index_flags >> 24;
+    r.m = q.m;
+    r.V = {q.Vx, q.Vy, q.Vz};
+    r.t = q.t; r.t1 = q.t1; r.Fdt = q.Fdt; r.last_step = q.last_step;
+    r.a = q.a; r.c = q.c; r.l = q.l;
+    r.tb = q.tb; r.dtb = q.dtb;
+    r.Ndt = r.Fdt / v.prm.nearMultiplier;          /* start_hit's expression */
+    r.dt = (r.flags & 1u) ? r.Ndt : r.Fdt;
+    r.P = add(f3(v.cam.C), mul(r.V, r.t));          /* march_step's expression */
+    r.k = 0;
+    r.launch_samples = 0;
+    r.phase = PH_MARCH;
+}
+
 struct PassArgs {
     uint32_t n_views;
     uint32_t K;                        /* lanes (samples) per ray in this pass */
-    const ContEntry *cont_in;          /* march_pass: the rays to advance */
+    const PassRec *cont_in;            /* march_pass: the rays to advance */
     const uint32_t *in_counts;         /* per view and sub-list, kHeadStride words apart */
-    ContEntry *cont_out;               /* the rays still marching */
+    PassRec *cont_out;                 /* the rays still marching */
     uint32_t *out_counts;
     RefineRec *refine_out;
     uint32_t *refine_counts;           /* per view and sub-region, kHeadStride words apart */
@@ -1561,13 +1590,20 @@ struct PassStash {
     float ca[64], aa[64];
 };
 
+/* Listed rays of view v's sub-list s. */
+__device__ __forceinline__ uint32_t cont_count(const ViewDesc *__restrict__ views, const uint32_t *counts,
+                                               uint32_t v, uint32_t s)
+{
+    return uniform(min(counts[(v * kContSubs + s) * kHeadStride], views[uniform(v)].cont_cap / kContSubs));
+}
+
 /* Appends the rays of this wave's lanes with `want` (leaders) to view vi's
  * sub-list of this block's XCD (block index modulo kContSubs), one atomic
  * per wave; rays that do not fit go on to the next sub-list. A view's
  * sub-lists hold all its rays together, and a full sub-list has every one of
  * its slots written, so each ray finds a slot within kContSubs tries. */
 __device__ __forceinline__ void append_cont(const ViewDesc &v, uint32_t vi, const Ray &r, bool want,
-                                            ContEntry *out, uint32_t *counts)
+                                            PassRec *out, uint32_t *counts)
 {
     const uint32_t lane = __lane_id();
     const uint32_t sub_cap = v.cont_cap / kContSubs;
@@ -1580,9 +1616,9 @@ __device__ __forceinline__ void append_cont(const ViewDesc &v, uint32_t vi, cons
         if (want) {
             const uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             if (slot < sub_cap) {
-                ContEntry c;
-                suspend_ray(r, c);
-                out[v.cont_base + sub * sub_cap + slot] = c;
+                PassRec q;
+                store_pass(r, q);
+                out[v.cont_base + sub * sub_cap + slot] = q;
                 want = false;
             }
         }
@@ -1638,9 +1674,9 @@ __global__ __launch_bounds__(256) void ray_start(const ViewDesc *__restrict__ vi
         const uint32_t sub_cap = v.cont_cap / kContSubs;
         const uint32_t slot = block_base + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
         if (slot < sub_cap) {
-            ContEntry c;
-            suspend_ray(r, c);
-            a.cont_out[v.cont_base + (blockIdx.x % kContSubs) * sub_cap + slot] = c;
+            PassRec q;
+            store_pass(r, q);
+            a.cont_out[v.cont_base + (blockIdx.x % kContSubs) * sub_cap + slot] = q;
             want = false;
         }
     }
@@ -1668,12 +1704,12 @@ __global__ __launch_bounds__(256) void march_pass(const ViewDesc *__restrict__ v
     const uint32_t pairs = a.n_views * kContSubs;
     uint32_t chunks = 0;
     for (uint32_t p = 0; p < pairs; ++p)
-        chunks += (cont_count(views, a.in_counts, kContSubs, p / kContSubs, p % kContSubs) + per - 1u) / per;
+        chunks += (cont_count(views, a.in_counts, p / kContSubs, p % kContSubs) + per - 1u) / per;
     uint32_t done_evals = 0;
     for (uint32_t c = wid; c < chunks; c += n_waves) {
         uint32_t acc = 0, p = 0, cnt = 0;
         for (; p < pairs; ++p) {
-            cnt = cont_count(views, a.in_counts, kContSubs, p / kContSubs, p % kContSubs);
+            cnt = cont_count(views, a.in_counts, p / kContSubs, p % kContSubs);
             const uint32_t nc = (cnt + per - 1u) / per;
             if (c < acc + nc) break;
             acc += nc;
@@ -1684,7 +1720,7 @@ __global__ __launch_bounds__(256) void march_pass(const ViewDesc *__restrict__ v
         bool busy = i < cnt;
         Ray r;
         r.phase = PH_IDLE;
-        if (busy) resume_ray(a.cont_in[v.cont_base + (p % kContSubs) * (v.cont_cap / kContSubs) + i], r);
+        if (busy) load_pass(v, a.cont_in[v.cont_base + (p % kContSubs) * (v.cont_cap / kContSubs) + i], r);
         /* `rounds` rounds (0: until every ray of the wave has ended); a ray
          * whose march ends leaves its group idle */
         for (uint32_t round = 0;; ++round) {
@@ -2241,6 +2277,8 @@ static PassPlan pass_plan(const LaunchPlan &plan, const ViewDesc *views, unsigne
     /* "K" or "KxR" per pass, comma separated */
     static const char *env = std::getenv("LYAP_MARCH_PASSES");
     const char *c = env ? env : "4,8,16,32,64,64,64x0";
+    /* (the last pass always runs its rays to the end: its records are the
+     * passes' own, which the persistent launches do not read) */
     if (c[0] == '0' && c[1] == 0) return pp;
     while (*c && pp.n < kMaxPasses) {
         char *e = nullptr;
@@ -2254,6 +2292,7 @@ static PassPlan pass_plan(const LaunchPlan &plan, const ViewDesc *views, unsigne
         if (*e != ',') break;
         c = e + 1;
     }
+    if (pp.n) pp.R[pp.n - 1] = 0;
     return pp;
 }
 
@@ -2414,7 +2453,7 @@ static int enqueue_passes(lyap_ctx *ctx, RenderResources &res, const ViewDesc *v
         cap_total += views[i].cont_cap;
         rays += (views[i].n_rays + 255u) / 256u * 256u;
     }
-    ContEntry *buf[2] = {(ContEntry *)res.cont_buf, (ContEntry *)res.cont_buf + cap_total};
+    PassRec *buf[2] = {(PassRec *)res.cont_buf, (PassRec *)res.cont_buf + cap_total};
     uint32_t *counts = (uint32_t *)res.cont_counts;
     const uint32_t pass_words = n_views * kContSubs * kHeadStride;
     const uint32_t cus = ctx->num_cus > 0 ? (uint32_t)ctx->num_cus : 256u;
@@ -2520,13 +2559,10 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
     uint32_t levels = (plan.direct || !has_rays) ? 1u : 4u;
     uint32_t first_level = 0;
     if (pp.n > 0) {
+        /* the passes march every ray to its end: no level launch follows */
         rc = enqueue_passes(ctx, res, views, n_views, pp, W, precision);
         if (rc) return rc;
-        /* the rays still marching after the last pass finish in one
-         * persistent launch (a later level: no budget, handed-over input) */
-        in_buf[3] = (pp.n & 1u) ? bufB : bufA;
-        /* a last pass that runs its rays to the end leaves none */
-        first_level = pp.R[pp.n - 1] == 0 ? levels : 3u;
+        first_level = levels;
     }
     for (uint32_t level = first_level; level < levels; ++level) {
         RenderArgs a{};
@@ -2543,10 +2579,7 @@ static int enqueue_views(lyap_ctx *ctx, RenderResources &res, const ViewDesc *vi
         a.direct = plan.direct ? 1u : 0u;
         a.resident_lanes = plan.resident_lanes;
         a.cont_in = in_buf[level];
-        a.in_counts = level ? (pp.n ? counts + pp.n * n_views * kContSubs * kHeadStride
-                                    : counts + (level - 1) * n_views)
-                            : nullptr;
-        a.in_subs = pp.n ? kContSubs : 0u;
+        a.in_counts = level ? counts + (level - 1) * n_views : nullptr;
         a.cont_out = out_buf[level];
         a.out_counts = level < 3 ? counts + level * n_views : nullptr;
         a.refine_out = park ? (RefineRec *)res.refine_buf : nullptr;
