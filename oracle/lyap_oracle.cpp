@@ -37,10 +37,12 @@
  *
  * Pinning: tests/test_oracle.py checks these functions against exact rational
  * hand computations, closed forms on the diagonal, permutation invariants,
- * an mpmath brute force, the analytic one-letter raymarch scene, and the
- * closed step recurrence of a constant-lambda ray (jitter, near step).
- * Parity unpinned: the order of the a/c sums of transparent samples
- * (kernel.cu:370-375) is restated from reading only.
+ * an mpmath brute force, the analytic one-letter raymarch scene, the
+ * closed step recurrence of a constant-lambda ray (jitter, near step), the
+ * a/c sums of a hit rebuilt sample by sample in march order
+ * (kernel.cu:370-375), the shading's cone, chaos and ambient branches, and
+ * the reversing walk's halving. Parity unpinned: RGBA of colour channels
+ * outside [0, 1] (undefined in the reference's conversion).
  */
 #include <cmath>
 #include <cstdint>
