@@ -1597,6 +1597,41 @@ __device__ __forceinline__ uint32_t cont_count(const ViewDesc *__restrict__ view
     return uniform(min(counts[(v * kContSubs + s) * kHeadStride], views[uniform(v)].cont_cap / kContSubs));
 }
 
+/* The chunks of up to 64 lists, held across a wave: lane j holds list j's
+ * entry count and the chunks of lists 0..j (an inclusive scan), so a chunk
+ * finds its list with one ballot. The counts come in one vector load rather
+ * than a walk of dependent scalar loads at every wave's start. */
+struct ChunkMap {
+    uint32_t cnt;      /* lane j: entries of list j (0 past the lists) */
+    uint32_t end;      /* lane j: chunks of lists 0..j */
+    uint32_t total;    /* chunks of all lists */
+};
+
+__device__ __forceinline__ ChunkMap chunk_map(uint32_t cnt, uint32_t per, uint32_t lane)
+{
+    ChunkMap m;
+    m.cnt = cnt;
+    uint32_t x = (cnt + per - 1u) / per;
+#pragma unroll
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+        if (lane >= off) x += y;
+    }
+    m.end = x;
+    m.total = uniform((uint32_t)__shfl((int)x, 63));
+    return m;
+}
+
+/* List p holding chunk c (< m.total), the chunks before p and p's entries.
+ * Every lane of the wave takes part. */
+__device__ __forceinline__ void chunk_find(const ChunkMap &m, uint32_t c, uint32_t per, uint32_t &p,
+                                           uint32_t &acc, uint32_t &cnt)
+{
+    p = uniform((uint32_t)__popcll(__ballot(m.end <= c)));
+    cnt = uniform((uint32_t)__shfl((int)m.cnt, (int)p));
+    acc = uniform((uint32_t)__shfl((int)m.end, (int)p)) - (cnt + per - 1u) / per;
+}
+
 /* Appends the rays of this wave's lanes with `want` (leaders) to view vi's
  * sub-list of this block's XCD (block index modulo kContSubs), one atomic
  * per wave; rays that do not fit go on to the next sub-list. A view's
@@ -1702,17 +1737,30 @@ __global__ __launch_bounds__(256) void march_pass(const ViewDesc *__restrict__ v
     const uint32_t n_waves = gridDim.x * 4u;
     const uint32_t wid = uniform(blockIdx.x * 4u + threadIdx.x / 64u);
     const uint32_t pairs = a.n_views * kContSubs;
+    const bool held = pairs <= 64u;
+    ChunkMap cm{0u, 0u, 0u};
     uint32_t chunks = 0;
-    for (uint32_t p = 0; p < pairs; ++p)
-        chunks += (cont_count(views, a.in_counts, p / kContSubs, p % kContSubs) + per - 1u) / per;
+    if (held) {
+        uint32_t n = 0;
+        if (lane < pairs) n = min(a.in_counts[lane * kHeadStride], views[lane / kContSubs].cont_cap / kContSubs);
+        cm = chunk_map(n, per, lane);
+        chunks = cm.total;
+    } else {
+        for (uint32_t p = 0; p < pairs; ++p)
+            chunks += (cont_count(views, a.in_counts, p / kContSubs, p % kContSubs) + per - 1u) / per;
+    }
     uint32_t done_evals = 0;
     for (uint32_t c = wid; c < chunks; c += n_waves) {
         uint32_t acc = 0, p = 0, cnt = 0;
-        for (; p < pairs; ++p) {
-            cnt = cont_count(views, a.in_counts, p / kContSubs, p % kContSubs);
-            const uint32_t nc = (cnt + per - 1u) / per;
-            if (c < acc + nc) break;
-            acc += nc;
+        if (held) {
+            chunk_find(cm, c, per, p, acc, cnt);
+        } else {
+            for (; p < pairs; ++p) {
+                cnt = cont_count(views, a.in_counts, p / kContSubs, p % kContSubs);
+                const uint32_t nc = (cnt + per - 1u) / per;
+                if (c < acc + nc) break;
+                acc += nc;
+            }
         }
         const uint32_t vi = uniform(p / kContSubs);
         const ViewDesc &v = views[vi];
@@ -1846,16 +1894,29 @@ __global__ __launch_bounds__(256) void refine_rays(const ViewDesc *__restrict__ 
     const uint32_t n_waves = gridDim.x * 4u;
     const uint32_t wid = uniform(blockIdx.x * 4u + threadIdx.x / 64u);
     const uint32_t pairs = a.n_views * kRefineSubs;
+    const bool held = pairs <= 64u;
+    ChunkMap cm{0u, 0u, 0u};
     uint32_t total = 0;
-    for (uint32_t p = 0; p < pairs; ++p) total += (refine_count(views, a, p) + 31u) / 32u;
+    if (held) {
+        uint32_t n = 0;
+        if (lane < pairs) n = min(a.counts[lane * kHeadStride], views[lane / kRefineSubs].refine_cap);
+        cm = chunk_map(n, 32u, lane);
+        total = cm.total;
+    } else {
+        for (uint32_t p = 0; p < pairs; ++p) total += (refine_count(views, a, p) + 31u) / 32u;
+    }
     unsigned long long done = 0;
     for (uint32_t t = wid; t < total; t += n_waves) {
         uint32_t acc = 0, p = 0, n = 0;
-        for (; p < pairs; ++p) {
-            n = refine_count(views, a, p);
-            const uint32_t nt = (n + 31u) / 32u;
-            if (t < acc + nt) break;
-            acc += nt;
+        if (held) {
+            chunk_find(cm, t, 32u, p, acc, n);
+        } else {
+            for (; p < pairs; ++p) {
+                n = refine_count(views, a, p);
+                const uint32_t nt = (n + 31u) / 32u;
+                if (t < acc + nt) break;
+                acc += nt;
+            }
         }
         const ViewDesc &v = views[uniform(p / kRefineSubs)];
         const uint32_t i = (t - acc) * 32u + (lane >> 1);
