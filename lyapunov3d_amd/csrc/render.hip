@@ -939,19 +939,63 @@ __device__ __forceinline__ uint32_t uniform(uint32_t x)
     return __builtin_amdgcn_readfirstlane(x);
 }
 
+/* The chunks of up to 64 lists, held across a wave: lane j holds list j's
+ * entry count and the chunks of lists 0..j (an inclusive scan), so a chunk
+ * finds its list with one ballot. The counts come in one vector load rather
+ * than a walk of dependent scalar loads at every wave's start. */
+struct ChunkMap {
+    uint32_t cnt;      /* lane j: entries of list j (0 past the lists) */
+    uint32_t end;      /* lane j: chunks of lists 0..j */
+    uint32_t total;    /* chunks of all lists */
+};
+
+__device__ __forceinline__ ChunkMap chunk_map(uint32_t cnt, uint32_t per, uint32_t lane)
+{
+    ChunkMap m;
+    m.cnt = cnt;
+    uint32_t x = (cnt + per - 1u) / per;
+#pragma unroll
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+        if (lane >= off) x += y;
+    }
+    m.end = x;
+    m.total = uniform((uint32_t)__shfl((int)x, 63));
+    return m;
+}
+
+/* List p holding chunk c (< m.total), the chunks before p and p's entries.
+ * Every lane of the wave takes part. */
+__device__ __forceinline__ void chunk_find(const ChunkMap &m, uint32_t c, uint32_t per, uint32_t &p,
+                                           uint32_t &acc, uint32_t &cnt)
+{
+    p = uniform((uint32_t)__popcll(__ballot(m.end <= c)));
+    cnt = uniform((uint32_t)__shfl((int)m.cnt, (int)p));
+    acc = uniform((uint32_t)__shfl((int)m.end, (int)p)) - (cnt + per - 1u) / per;
+}
+
 /* Rays [first, last) of view `view` for tile t. Level 0 tiles cover the
  * views' pixel blocks; later levels cover each view's handed-over rays,
- * `tile` entries per tile. Views are few: scalar scans. Returns false when t
- * is past the last tile. */
+ * `tile` entries per tile. Up to 64 views: lane v reads view v and one
+ * ballot finds the view; more: scalar scans. Every lane of the wave takes
+ * part. Returns false when t is past the last tile. */
 __device__ __forceinline__ bool tile_lookup(const ViewDesc *__restrict__ views,
                                             const RenderArgs &a, uint32_t t,
                                             uint32_t tile, uint32_t &view,
                                             uint32_t &first, uint32_t &last)
 {
+    const uint32_t lane = __lane_id();
+    const bool held = a.n_views <= 64u;
     if (a.level == 0) {
         if (t >= a.total_tiles) return false;
         uint32_t v = 0;
-        while (v + 1 < a.n_views && views[uniform(v + 1)].tile_begin <= t) ++v;
+        if (held) {
+            /* tile_begin rises with the view index and view 0 begins at 0 */
+            const bool before = lane < a.n_views && views[lane].tile_begin <= t;
+            v = (uint32_t)__popcll(__ballot(before)) - 1u;
+        } else {
+            while (v + 1 < a.n_views && views[uniform(v + 1)].tile_begin <= t) ++v;
+        }
         view = uniform(v);
         const ViewDesc &d = views[view];
         const uint32_t len = uniform(d.tile_len);
@@ -1102,8 +1146,15 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
         /* size the groups to the number of rays handed over: samples per
          * ray and round as lanes would be if each took one */
         uint32_t total = 0;
-        for (uint32_t v = 0; v < a.n_views; ++v)
-            total += uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+        if (a.n_views <= 64u) {
+            uint32_t n = 0;
+            if (__lane_id() < a.n_views) n = min(a.in_counts[__lane_id()], views[__lane_id()].cont_cap);
+            for (int off = 32; off > 0; off >>= 1) n += (uint32_t)__shfl_xor((int)n, off);
+            total = uniform(n);
+        } else {
+            for (uint32_t v = 0; v < a.n_views; ++v)
+                total += uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+        }
         if (total == 0) return;
         K0 = 1;
         while (K0 < 64u && (uint64_t)a.resident_lanes >= (uint64_t)total * (2u * K0)) K0 *= 2;
@@ -1119,9 +1170,17 @@ void render_persistent(const ViewDesc *__restrict__ views, RenderArgs a)
     uint32_t level_tiles = a.total_tiles;
     if (a.level > 0) {
         level_tiles = 0;
-        for (uint32_t v = 0; v < a.n_views; ++v) {
-            const uint32_t cnt = uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
-            level_tiles += (cnt + tile - 1u) / tile;
+        if (a.n_views <= 64u) {
+            uint32_t n = 0;
+            if (__lane_id() < a.n_views) n = min(a.in_counts[__lane_id()], views[__lane_id()].cont_cap);
+            n = (n + tile - 1u) / tile;
+            for (int off = 32; off > 0; off >>= 1) n += (uint32_t)__shfl_xor((int)n, off);
+            level_tiles = uniform(n);
+        } else {
+            for (uint32_t v = 0; v < a.n_views; ++v) {
+                const uint32_t cnt = uniform(min(a.in_counts[v], views[uniform(v)].cont_cap));
+                level_tiles += (cnt + tile - 1u) / tile;
+            }
         }
     }
     uint32_t K = K0;                  /* lanes per ray; wave-uniform */
@@ -1595,41 +1654,6 @@ __device__ __forceinline__ uint32_t cont_count(const ViewDesc *__restrict__ view
                                                uint32_t v, uint32_t s)
 {
     return uniform(min(counts[(v * kContSubs + s) * kHeadStride], views[uniform(v)].cont_cap / kContSubs));
-}
-
-/* The chunks of up to 64 lists, held across a wave: lane j holds list j's
- * entry count and the chunks of lists 0..j (an inclusive scan), so a chunk
- * finds its list with one ballot. The counts come in one vector load rather
- * than a walk of dependent scalar loads at every wave's start. */
-struct ChunkMap {
-    uint32_t cnt;      /* lane j: entries of list j (0 past the lists) */
-    uint32_t end;      /* lane j: chunks of lists 0..j */
-    uint32_t total;    /* chunks of all lists */
-};
-
-__device__ __forceinline__ ChunkMap chunk_map(uint32_t cnt, uint32_t per, uint32_t lane)
-{
-    ChunkMap m;
-    m.cnt = cnt;
-    uint32_t x = (cnt + per - 1u) / per;
-#pragma unroll
-    for (uint32_t off = 1; off < 64u; off <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)x, off);
-        if (lane >= off) x += y;
-    }
-    m.end = x;
-    m.total = uniform((uint32_t)__shfl((int)x, 63));
-    return m;
-}
-
-/* List p holding chunk c (< m.total), the chunks before p and p's entries.
- * Every lane of the wave takes part. */
-__device__ __forceinline__ void chunk_find(const ChunkMap &m, uint32_t c, uint32_t per, uint32_t &p,
-                                           uint32_t &acc, uint32_t &cnt)
-{
-    p = uniform((uint32_t)__popcll(__ballot(m.end <= c)));
-    cnt = uniform((uint32_t)__shfl((int)m.cnt, (int)p));
-    acc = uniform((uint32_t)__shfl((int)m.end, (int)p)) - (cnt + per - 1u) / per;
 }
 
 /* Appends the rays of this wave's lanes with `want` (leaders) to view vi's
