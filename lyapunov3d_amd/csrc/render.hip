@@ -976,8 +976,9 @@ __device__ __forceinline__ void chunk_find(const ChunkMap &m, uint32_t c, uint32
 
 /* Rays [first, last) of view `view` for tile t. Level 0 tiles cover the
  * views' pixel blocks; later levels cover each view's handed-over rays,
- * `tile` entries per tile. Up to 64 views: lane v reads view v and one
- * ballot finds the view; more: scalar scans. Every lane of the wave takes
+ * `tile` entries per tile. Level 0 with up to 64 views: lane v reads view
+ * v's first tile and one ballot finds the view; otherwise scalar scans (a
+ * held scan here costs the kernel spills). Every lane of the wave takes
  * part. Returns false when t is past the last tile. */
 __device__ __forceinline__ bool tile_lookup(const ViewDesc *__restrict__ views,
                                             const RenderArgs &a, uint32_t t,
