@@ -13,6 +13,7 @@
    the same bytes.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -262,6 +263,35 @@ def test_march_passes_same_bytes(ctx, prec, margin):
         ctx.render_batch(as_api(views, bufs, ld, n_lights), m)
         ctx.sync()
         ref.append((ctx.last_evals(), [b.fetch() for b in bufs]))
+    assert ref[0][0] == ref[1][0]
+    for a, b in zip(ref[0][1], ref[1][1]):
+        same(a, b)
+
+
+def test_march_passes_many_views_same_bytes(ctx):
+    """Twelve views make 96 sub-lists (eight per view), more than a wave
+    holds, so the passes and the refinement launch find their chunks by
+    walking the counts: the batch marches in passes (nine launches) and gives
+    the persistent levels' bytes and evaluation count."""
+    views, lights, n_lights = small_batch(12, seed=9, sides=(24, 40, 17))
+    ld = device_lights(lights, n_lights)
+    passes = W.c2_mode(fp64_margin=0.0)
+    levels = W.c2_mode(fp64_margin=0.0)
+    levels.schedule = lyap.SCHEDULE_LEVELS
+    ref = []
+    for m in (passes, levels):
+        bufs = alloc(views)
+        ctx.timing_collect()
+        ctx.timing(True)
+        try:
+            ctx.render_batch(as_api(views, bufs, ld, n_lights), m)
+            ctx.sync()
+            launches = ctx.timing_collect()[1]
+        finally:
+            ctx.timing(False)
+        ref.append((ctx.last_evals(), [b.fetch() for b in bufs], launches))
+    if os.environ.get("LYAP_MARCH_PASSES") != "0":
+        assert ref[0][2] == 9
     assert ref[0][0] == ref[1][0]
     for a, b in zip(ref[0][1], ref[1][1]):
         same(a, b)
